@@ -1,0 +1,234 @@
+/* C ABI of the MI355X-native RCWA solve path (librcwa_amd.so).
+ *
+ * Drop-in boundary for rcwa3d (LukhaGian/rcwa-cuda).  The reference has no
+ * plugin layer: its interface is the header rcwa3d/include/rcwa.hpp, whose
+ * stage functions are called in sequence by rcwa3d/src/main.cpp:126-191.
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions shared with the reference:
+ *   - complex numbers are (re, im) double pairs, the layout of
+ *     std::complex<double> (rcwa.hpp:12);
+ *   - matrices are column-major (rcwa.hpp:13);
+ *   - permittivity/permeability grids are [layer][iy][ix] (rcwa.hpp:56);
+ *   - harmonic index m = ix*Q + iy, x slow, y fast (rcwa.cpp:63-64);
+ *   - R and T are Q x P, column-major, rows indexed by the y order
+ *     (rcwa.cpp:576, 582);
+ *   - angles are in radians, lengths in any consistent unit.
+ * Differences by design: nothing aborts or prints (rcwa.cpp:300 aborts);
+ * every call returns an rcwa_status and rcwa_last_error() describes the
+ * last failure of the calling thread.  No memory allocated by the library
+ * crosses the ABI: host outputs are caller-owned, device memory is owned by
+ * the context or server.
+ */
+#ifndef RCWA_AMD_H
+#define RCWA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } rcwa_c128;
+
+typedef enum {
+  RCWA_OK = 0,
+  RCWA_E_ARG = 1,         /* invalid argument or shape */
+  RCWA_E_NOMEM = 2,       /* request exceeds the context's HBM budget */
+  RCWA_E_EIG_NOCONV = 3,  /* QR iteration did not converge (rcwa.cpp:300 aborts here) */
+  RCWA_E_SINGULAR = 4,    /* an exactly singular matrix was inverted */
+  RCWA_E_NONFINITE = 5,   /* non-finite values in inputs or results */
+  RCWA_E_HIP = 6,         /* HIP runtime error */
+  RCWA_E_NOTREADY = 7,    /* server ticket not finished yet */
+  RCWA_E_CANCELLED = 8    /* server ticket cancelled */
+} rcwa_status;
+
+typedef enum { RCWA_FP64 = 0, RCWA_SPLIT = 1 } rcwa_precision;
+
+/* rcwa.hpp:48-78 (struct Device).  er/ur point to num_layers*Ny*Nx complex
+ * values, [layer][iy][ix]; t to num_layers thicknesses.  When
+ * grids_on_device is non-zero, er and ur are device pointers (HBM-resident
+ * inputs); otherwise host pointers. */
+typedef struct {
+  int Nx, Ny, num_layers;
+  double Lx, Ly;
+  const rcwa_c128* er;
+  const rcwa_c128* ur;
+  const double* t;
+  int grids_on_device;
+} rcwa_device;
+
+/* rcwa.hpp:81-95 (struct Source); k0 = 2 pi / lambda0 is derived. */
+typedef struct {
+  double lambda0, theta, phi, pte, ptm;
+} rcwa_source;
+
+/* rcwa.hpp:98-112 (struct RCWAParams). */
+typedef struct {
+  int Nx_harmonics, Ny_harmonics;
+  rcwa_c128 er_ref, ur_ref, er_trn, ur_trn;
+} rcwa_params;
+
+/* rcwa.hpp:116-128 (struct Results).  R and T are caller-allocated arrays of
+ * (2*Ny_harmonics+1)*(2*Nx_harmonics+1) doubles (Q x P, column-major). */
+typedef struct {
+  double* R;
+  double* T;
+  double R_tot, T_tot;
+} rcwa_results;
+
+/* One independent solve: a device, its media and nsrc sources.  Sources
+ * with equal (lambda0, theta, phi) share one S-matrix (polarisation enters
+ * only at rcwa.cpp:495); each distinct incidence gets its own. */
+typedef struct {
+  const rcwa_device* device;
+  const rcwa_params* params;
+  const rcwa_source* src;
+  int nsrc;
+} rcwa_problem;
+
+/* Per-call statistics (stage times from HIP events). */
+typedef struct {
+  double ms_total, ms_convmat, ms_layers, ms_eig, ms_fold, ms_out;
+  double flops_algorithmic; /* 2384 N^3 per layer + 1621 N^3 per S-matrix */
+  int n_smatrices, n_layers;
+} rcwa_stats;
+
+typedef struct rcwa_ctx rcwa_ctx;
+
+const char* rcwa_last_error(void);
+const char* rcwa_version(void);
+
+/* Context: one per host thread; owns a HIP stream, a device memory pool of at
+ * most hbm_budget bytes (0 = 80% of free HBM) and the eigensolver limits. */
+rcwa_status rcwa_ctx_create(int device, size_t hbm_budget, rcwa_ctx** out);
+rcwa_status rcwa_ctx_destroy(rcwa_ctx* ctx);
+/* QR sweep limit per eigenvalue (default 30, as Eigen's ComplexSchur). */
+rcwa_status rcwa_ctx_set_eig_maxit(rcwa_ctx* ctx, int maxit_per_row);
+rcwa_status rcwa_ctx_get_stats(const rcwa_ctx* ctx, rcwa_stats* out);
+rcwa_status rcwa_ctx_reset_stats(rcwa_ctx* ctx);
+/* Upper bound on the number of S-matrices batched into one chunk (0 = as
+ * many as the HBM budget allows). */
+rcwa_status rcwa_ctx_set_chunk(rcwa_ctx* ctx, int max_chunk);
+
+/* Full solve, replacing the main.cpp:126-191 sequence (ConvMat per layer ->
+ * ComputeWaveVectors -> GapMedium -> SMatrixInit -> per-layer SMatrixLayer
+ * and RedhefferProduct -> SMatrixReflection/SMatrixTransmission -> two
+ * RedhefferProducts -> ComputeSourceModeCoeff -> Compute{Reflected,
+ * Transmitted}Field -> ComputeDiffractionEfficiencies).  out has nsrc
+ * entries. */
+rcwa_status rcwa_solve(rcwa_ctx* ctx, const rcwa_device* dev, const rcwa_params* params,
+                       const rcwa_source* src, int nsrc, rcwa_precision precision,
+                       rcwa_results* out);
+
+/* Batched form of rcwa_solve over independent problems (e.g. a spectral
+ * sweep).  out is laid out problem by problem, nsrc entries each. */
+rcwa_status rcwa_solve_batch(rcwa_ctx* ctx, const rcwa_problem* probs, int nprob,
+                             rcwa_precision precision, rcwa_results* out);
+
+/* Optional: global S-matrix blocks S11, S21 columns for the two source
+ * entries (indices PQ/2 and PQ/2+PQ) of the last solve_batch, for tests.
+ * cols: nprob_smatrices x 4 x (2N) complex. */
+rcwa_status rcwa_last_source_columns(rcwa_ctx* ctx, rcwa_c128* cols, size_t count);
+
+/* ----- stage entry points (host buffers), one per reference function ----- */
+
+/* rcwa.hpp:172 ConvMat: out is PQ x PQ column-major for layer `layer`. */
+rcwa_status rcwa_convmat(rcwa_ctx* ctx, const rcwa_c128* field, int layer, int Nx, int Ny,
+                         int Nx_harmonics, int Ny_harmonics, rcwa_c128* out);
+
+/* rcwa.hpp:175 ComputeWaveVectors: k_inc[3], Kx, Ky, Kz_ref, Kz_trn [PQ]. */
+rcwa_status rcwa_wave_vectors(double Lx, double Ly, const rcwa_source* src,
+                              const rcwa_params* params, rcwa_c128* k_inc, rcwa_c128* Kx,
+                              rcwa_c128* Ky, rcwa_c128* Kz_ref, rcwa_c128* Kz_trn);
+
+/* rcwa.hpp:178 GapMedium: W0, V0 are 2PQ x 2PQ column-major. */
+rcwa_status rcwa_gap_medium(int PQ, const rcwa_c128* Kx, const rcwa_c128* Ky, rcwa_c128* W0,
+                            rcwa_c128* V0);
+
+/* rcwa.hpp:184 SMatrixLayer for `batch` layers that share Kx, Ky (one
+ * incidence): erc/urc are batch x (PQ x PQ), k0t[b] = k0 * t_b; outputs S11
+ * and S12 batch x (2PQ x 2PQ) (S22 = S11, S21 = S12, rcwa.cpp:351). */
+rcwa_status rcwa_layer_smatrix(rcwa_ctx* ctx, int PQ, int batch, const rcwa_c128* erc,
+                               const rcwa_c128* urc, const rcwa_c128* Kx, const rcwa_c128* Ky,
+                               const double* k0t, rcwa_c128* S11, rcwa_c128* S12);
+
+/* rcwa.hpp:190/193 SMatrixReflection (which = 0) / SMatrixTransmission
+ * (which = 1): four 2PQ x 2PQ blocks S11, S12, S21, S22; W_end = I. */
+rcwa_status rcwa_end_smatrix(int which, int PQ, const rcwa_params* params, const rcwa_c128* Kx,
+                             const rcwa_c128* Ky, const rcwa_c128* Kz, rcwa_c128* S11,
+                             rcwa_c128* S12, rcwa_c128* S21, rcwa_c128* S22);
+
+/* rcwa.hpp:214 RedhefferProduct on `batch` pairs: A[4], B[4], S[4] each
+ * point to batch x (n x n) blocks in the order S11, S12, S21, S22. */
+rcwa_status rcwa_star_product(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* const* A,
+                              const rcwa_c128* const* B, rcwa_c128* const* S);
+
+/* rcwa.hpp:233 extraction(ComplexEigenSolver) on `batch` n x n matrices:
+ * eigenvalues w (batch x n) and unit-norm eigenvectors V (batch x n x n). */
+rcwa_status rcwa_eig(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* A, rcwa_c128* w,
+                     rcwa_c128* V);
+
+/* Batched inverse (the LU solves of the reference, rcwa.cpp:267-281). */
+rcwa_status rcwa_inverse(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* A, rcwa_c128* Ainv);
+
+/* C = A * B for batch column-major matrices (the Eigen products). */
+rcwa_status rcwa_zgemm(rcwa_ctx* ctx, int m, int n, int k, int batch, const rcwa_c128* A,
+                       const rcwa_c128* B, rcwa_c128* C);
+/* The same product in the split-precision form (FP16 slices on the matrix
+ * cores, FP32 accumulation); no counterpart in the reference. */
+rcwa_status rcwa_zgemm_split(rcwa_ctx* ctx, int m, int n, int k, int batch, const rcwa_c128* A,
+                             const rcwa_c128* B, rcwa_c128* C);
+
+/* rcwa.hpp:200-209 ComputeSourceModeCoeff + ComputeReflectedField +
+ * ComputeTransmittedField + ComputeDiffractionEfficiencies from the
+ * S11/S21 columns at the two source entries (W_ref = W_trn = I):
+ * s11c, s21c are 2 x 2PQ complex (column PQ/2 then PQ/2+PQ). */
+rcwa_status rcwa_efficiencies(const rcwa_source* src, const rcwa_params* params,
+                              const rcwa_c128* k_inc, const rcwa_c128* Kx, const rcwa_c128* Ky,
+                              const rcwa_c128* Kz_ref, const rcwa_c128* Kz_trn,
+                              const rcwa_c128* s11c, const rcwa_c128* s21c, rcwa_results* out);
+
+/* ----- device-pointer batched stage entry points (HBM-resident) ----- */
+
+/* Convolution matrices for `batch` grids [b][iy][ix] (device) -> C (device,
+ * batch x PQ x PQ).  Enqueued on the context stream. */
+rcwa_status rcwa_convmat_batched(rcwa_ctx* ctx, const rcwa_c128* grids_dev, int Nx, int Ny,
+                                 int Nx_harmonics, int Ny_harmonics, int batch,
+                                 rcwa_c128* C_dev);
+
+/* ----- serving form (continuous batching) ----- */
+
+typedef struct rcwa_server rcwa_server;
+
+typedef struct {
+  size_t hbm_budget;     /* bytes of HBM the server may hold (0 = 60% of free) */
+  int max_batch;         /* largest batched launch (problems), 0 = auto */
+  int n_workers;         /* concurrent GPU streams, 0 = 2 */
+  rcwa_precision precision;
+} rcwa_server_config;
+
+typedef struct {
+  double t_submit, t_start, t_finish; /* seconds on the server clock */
+  int batch_size;                     /* size of the launch it ran in */
+} rcwa_ticket_info;
+
+rcwa_status rcwa_server_create(int device, const rcwa_server_config* cfg, rcwa_server** out);
+/* Thread-safe.  Copies the problem's host data; the caller may reuse its
+ * buffers as soon as this returns. */
+rcwa_status rcwa_server_submit(rcwa_server* srv, const rcwa_problem* prob, uint64_t* ticket);
+/* Non-blocking: RCWA_E_NOTREADY until done; then fills out (nsrc entries). */
+rcwa_status rcwa_server_poll(rcwa_server* srv, uint64_t ticket, rcwa_results* out,
+                             rcwa_ticket_info* info);
+rcwa_status rcwa_server_wait(rcwa_server* srv, uint64_t ticket, rcwa_results* out,
+                             rcwa_ticket_info* info);
+rcwa_status rcwa_server_cancel(rcwa_server* srv, uint64_t ticket);
+double rcwa_server_clock(rcwa_server* srv);
+rcwa_status rcwa_server_destroy(rcwa_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RCWA_AMD_H */

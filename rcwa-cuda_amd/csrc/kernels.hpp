@@ -1,0 +1,69 @@
+// Launchers for the gfx950 kernels of the RCWA solve path.  Every launcher
+// takes the stream it enqueues on and returns 0 or a hipError_t value.
+#pragma once
+#include "common.hpp"
+
+// zgemm.hip ------------------------------------------------------------------
+int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                  ZCMat D, ZMat C, int batch);
+
+// linalg.hip -----------------------------------------------------------------
+// In-place inverse of a batch of n x n matrices by blocked Gauss-Jordan
+// elimination with partial (row) pivoting.  ws must hold
+// gj_workspace_bytes(n, batch) bytes.  info[b] is set to 1 when a pivot is
+// exactly zero (the matrix is singular); it is not cleared.
+size_t gj_workspace_bytes(int n, int batch);
+int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* info);
+
+// Elementwise helpers.
+int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag);
+int zcopy_batched(hipStream_t s, int rows, int cols, ZCMat src, ZMat dst, int batch);
+// M(i,j) = r[b*rstride + i] * M(i,j) * c[b*cstride + j]; r or c may be null.
+int zscale_rc(hipStream_t s, int rows, int cols, ZMat M, const zc* r, long long rstride,
+              const zc* c, long long cstride, int batch);
+// dst = src * diag(c) (column scaling) into a separate buffer.
+int zscale_cols_to(hipStream_t s, int rows, int cols, ZCMat src, const zc* c, long long cstride,
+                   ZMat dst, int batch);
+// Normalise every column to unit 2-norm.
+int znormalize_cols(hipStream_t s, int n, ZMat M, int batch);
+// Count non-finite entries of `count` complex values (adds to *flag).
+int zcount_nonfinite(hipStream_t s, const zc* p, long long count, int* flag);
+
+// eig.hip --------------------------------------------------------------------
+// Eigen-decomposition of a batch of general complex matrices:
+// A (overwritten by its Schur factor) -> eigenvalues w[b*n + i] and unit
+// 2-norm eigenvectors W.  ws must hold eig_workspace_bytes(n, batch) bytes.
+// info[b] = 1 if the QR iteration did not converge within maxit_per_row*n
+// sweeps.
+size_t eig_workspace_bytes(int n, int batch);
+int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
+                int maxit_per_row);
+// The three phases, exposed for tests.
+int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
+int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row);
+int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch);
+
+// rcwa_kernels.hip -----------------------------------------------------------
+// Truncated 2-D DFT of grids [b][iy][ix] at frequencies |fx| <= 2hx,
+// |fy| <= 2hy, normalised by 1/(Nx Ny): F[b][fy+2hy][fx+2hx].
+int convmat_dft(hipStream_t s, const zc* grids, int Nx, int Ny, int hx, int hy, zc* F,
+                int batch, zc* rowbuf);
+size_t convmat_rowbuf_elems(int Nx, int Ny, int hx, int batch);
+// Toeplitz gather C[b](m, n) = F[b](dq, dp) (rcwa.cpp:56-78), zero outside
+// the grid's frequency range.
+int convmat_gather(hipStream_t s, const zc* F, int Nx, int Ny, int hx, int hy, ZMat C,
+                   int batch);
+// P and Q of rcwa.cpp:264-285 from the inverses of [[eps]] and [[mu]].
+// Wave vectors Kx, Ky are per batch entry (stride N).
+int build_pq(hipStream_t s, int N, ZCMat erc, ZCMat urc, ZCMat erci, ZCMat urci, const zc* Kx,
+             const zc* Ky, ZMat P, ZMat Q, int batch);
+// lam = usqrt(lam2), ilam = 1/lam, X = exp(-k0 t lam) (rcwa.cpp:317, 339).
+int layer_modes(hipStream_t s, int n, const zc* lam2, zc* lam, zc* ilam, zc* X,
+                const double* k0t, int batch);
+// A = Wi + Vi V0, B = Wi - Vi V0 for the 2x2-block-diagonal V0 of the gap
+// medium, given by its four diagonals v0[b][4][N] (rcwa.cpp:335-338).
+int build_ab(hipStream_t s, int N, ZCMat Wi, ZCMat Vi, const zc* v0, ZMat A, ZMat B, int batch);
+// Dense n x n matrix from a 2x2 block structure of diagonals d + b*dstride, [4][N].
+int expand_block_diag(hipStream_t s, int N, const zc* d, long long dstride, ZMat M, int batch);
+// Gather columns cols[0..ncol) of M into out[b][c][n].
+int gather_cols(hipStream_t s, int n, ZCMat M, const int* cols, int ncol, zc* out, int batch);

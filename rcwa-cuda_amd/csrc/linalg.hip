@@ -1,0 +1,284 @@
+// Batched dense complex linear algebra besides GEMM: the blocked
+// Gauss-Jordan inverse that replaces the reference's PartialPivLU solves
+// (rcwa.cpp:267-281, 335-348, 605-611) and small elementwise kernels.
+//
+// Gauss-Jordan in place, blocked by nb columns.  One panel step:
+//   1. the panel kernel runs nb unblocked Gauss-Jordan steps on the n x nb
+//      column panel: pivot search in the not-yet-pivoted rows, a row swap
+//      applied to all n columns, and elimination restricted to the panel;
+//      afterwards the panel holds the nb columns G of the accumulated
+//      elementary transform E = I + (G - I_K) e_K^T;
+//   2. the other columns receive E as one rank-nb update,
+//      M(:, j) += (G - I_K) M(K, j)  for j outside the panel,
+//      which is a ZGEMM on the matrix cores.
+// Row swaps are undone at the end as column swaps in reverse order, so the
+// result is A^-1 (not (PA)^-1).
+#include <algorithm>
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+constexpr int GJ_NB = 32;
+constexpr int GJ_THREADS = 512;
+
+struct ArgMax {
+  double v;
+  int i;
+};
+
+__device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b) {
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+__global__ __launch_bounds__(GJ_THREADS) void gj_panel_kernel(int n, int k0, int nb, ZMat Mm,
+                                                              int* perm, int* info) {
+  __shared__ zc rowc[GJ_NB];
+  __shared__ ArgMax red[GJ_THREADS / 64];
+  __shared__ int s_piv;
+  __shared__ zc s_inv;
+  const int b = blockIdx.x;
+  zc* A = Mm.p + (long long)b * Mm.stride;
+  const long long ld = Mm.ld;
+  int* pv = perm + (long long)b * n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int kend = min(n, k0 + nb);
+  for (int c = k0; c < kend; ++c) {
+    // 1. pivot: largest |A(i,c)|^2 over rows c..n-1 (first index on ties)
+    ArgMax best{-1.0, n};
+    for (int i = c + tid; i < n; i += GJ_THREADS) {
+      ArgMax cand{zabs2(A[i + c * ld]), i};
+      best = better(best, cand);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      ArgMax o{__shfl_down(best.v, off), __shfl_down(best.i, off)};
+      best = better(best, o);
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    if (tid == 0) {
+      ArgMax r = red[0];
+      for (int w = 1; w < GJ_THREADS / 64; ++w) r = better(r, red[w]);
+      int p = (r.i < n) ? r.i : c;
+      if (!(r.v > 0.0)) {
+        if (info) info[b] = 1;
+        p = c;
+      }
+      s_piv = p;
+      pv[c] = p;
+    }
+    __syncthreads();
+    const int p = s_piv;
+    // 2. swap rows c and p across all columns
+    if (p != c) {
+      for (int j = tid; j < n; j += GJ_THREADS) {
+        zc t = A[c + j * ld];
+        A[c + j * ld] = A[p + j * ld];
+        A[p + j * ld] = t;
+      }
+    }
+    __syncthreads();
+    // 3. scaled pivot row within the panel
+    if (tid == 0) s_inv = zinv(A[c + c * ld]);
+    __syncthreads();
+    const zc inv = s_inv;
+    if (tid < kend - k0) rowc[tid] = zmul(A[c + (k0 + tid) * ld], inv);
+    __syncthreads();
+    // 4. eliminate column c from every other row of the panel
+    for (int i = tid; i < n; i += GJ_THREADS) {
+      const zc aic = A[i + c * ld];
+      if (i == c) {
+        for (int j = k0; j < kend; ++j)
+          A[c + j * ld] = (j == c) ? inv : rowc[j - k0];
+      } else {
+        for (int j = k0; j < kend; ++j) {
+          if (j == c) continue;
+          A[i + j * ld] = zsub(A[i + j * ld], zmul(aic, rowc[j - k0]));
+        }
+        A[i + c * ld] = zneg(zmul(aic, inv));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Gp(i, r) = A(i, k0+r) - delta(i, k0+r);  Tm(r, j) = A(k0+r, j), 0 on the panel.
+__global__ void gj_prep_kernel(int n, int k0, int nb, ZCMat Mm, zc* Gp, zc* Tm) {
+  const int b = blockIdx.y;
+  const zc* A = Mm.p + (long long)b * Mm.stride;
+  const long long ld = Mm.ld;
+  zc* G = Gp + (long long)b * n * nb;
+  zc* T = Tm + (long long)b * n * nb;
+  const int kend = min(n, k0 + nb), w = kend - k0;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * w;
+       idx += gridDim.x * blockDim.x) {
+    const int i = idx % n, r = idx / n;
+    zc g = A[i + (long long)(k0 + r) * ld];
+    if (i == k0 + r) g.x -= 1.0;
+    G[i + (long long)r * n] = g;
+    const int j = i;  // reuse the index pair for the row block: (r, j)
+    zc t = (j >= k0 && j < kend) ? zmake(0, 0) : A[(k0 + r) + (long long)j * ld];
+    T[r + (long long)j * w] = t;
+  }
+}
+
+__global__ __launch_bounds__(256) void gj_unswap_kernel(int n, ZMat Mm, const int* perm) {
+  const int b = blockIdx.x;
+  zc* A = Mm.p + (long long)b * Mm.stride;
+  const long long ld = Mm.ld;
+  const int* pv = perm + (long long)b * n;
+  for (int c = n - 1; c >= 0; --c) {
+    const int p = pv[c];
+    if (p != c) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        zc t = A[i + c * ld];
+        A[i + c * ld] = A[i + p * ld];
+        A[i + p * ld] = t;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void identity_kernel(int n, ZMat M, zc diag) {
+  const int b = blockIdx.y;
+  zc* A = M.p + (long long)b * M.stride;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)n * n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    A[i + j * M.ld] = (i == j) ? diag : zmake(0, 0);
+  }
+}
+
+__global__ void copy_kernel(int rows, int cols, ZCMat S, ZMat D) {
+  const int b = blockIdx.y;
+  const zc* s = S.p + (long long)b * S.stride;
+  zc* d = D.p + (long long)b * D.stride;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       idx < (long long)rows * cols; idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % rows), j = (int)(idx / rows);
+    d[i + j * D.ld] = s[i + j * S.ld];
+  }
+}
+
+__global__ void scale_rc_kernel(int rows, int cols, ZMat M, const zc* r, long long rs,
+                                const zc* c, long long cs) {
+  const int b = blockIdx.y;
+  zc* A = M.p + (long long)b * M.stride;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       idx < (long long)rows * cols; idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % rows), j = (int)(idx / rows);
+    zc v = A[i + j * M.ld];
+    if (r) v = zmul(r[b * rs + i], v);
+    if (c) v = zmul(v, c[b * cs + j]);
+    A[i + j * M.ld] = v;
+  }
+}
+
+__global__ void scale_cols_to_kernel(int rows, int cols, ZCMat S, const zc* c, long long cs,
+                                     ZMat D) {
+  const int b = blockIdx.y;
+  const zc* s = S.p + (long long)b * S.stride;
+  zc* d = D.p + (long long)b * D.stride;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       idx < (long long)rows * cols; idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % rows), j = (int)(idx / rows);
+    d[i + j * D.ld] = zmul(s[i + j * S.ld], c[b * cs + j]);
+  }
+}
+
+__global__ __launch_bounds__(256) void normalize_cols_kernel(int n, ZMat M) {
+  __shared__ double red[4];
+  const int j = blockIdx.x, b = blockIdx.y;
+  zc* col = M.p + (long long)b * M.stride + (long long)j * M.ld;
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += zabs2(col[i]);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const double tot = red[0] + red[1] + red[2] + red[3];
+  const double inv = tot > 0 ? 1.0 / sqrt(tot) : 1.0;
+  for (int i = threadIdx.x; i < n; i += 256) col[i] = zscale(col[i], inv);
+}
+
+__global__ void nonfinite_kernel(const zc* p, long long count, int* flag) {
+  int bad = 0;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < count;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const zc v = p[idx];
+    if (!isfinite(v.x) || !isfinite(v.y)) bad = 1;
+  }
+  if (bad) atomicAdd(flag, 1);
+}
+
+inline dim3 grid_for(long long elems, int batch) {
+  long long blocks = (elems + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  return dim3((unsigned)blocks, (unsigned)batch);
+}
+
+}  // namespace
+
+size_t gj_workspace_bytes(int n, int batch) {
+  const size_t nb = GJ_NB;
+  return (size_t)batch * (2 * n * nb * sizeof(zc) + n * sizeof(int)) + 256;
+}
+
+int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* info) {
+  if (n <= 0 || batch <= 0) return 0;
+  zc* Gp = (zc*)ws;
+  zc* Tm = Gp + (size_t)batch * n * GJ_NB;
+  int* perm = (int*)(Tm + (size_t)batch * n * GJ_NB);
+  for (int k0 = 0; k0 < n; k0 += GJ_NB) {
+    const int w = std::min(GJ_NB, n - k0);
+    hipLaunchKernelGGL(gj_panel_kernel, dim3(batch), dim3(GJ_THREADS), 0, s, n, k0, w, M, perm,
+                       info);
+    if (k0 + w >= n && k0 == 0) break;  // single panel covers the matrix
+    hipLaunchKernelGGL(gj_prep_kernel, grid_for((long long)n * w, batch), dim3(256), 0, s, n, k0,
+                       w, ZCMat{M.p, M.ld, M.stride}, Gp, Tm);
+    int e = zgemm_batched(s, n, n, w, zmake(1, 0), ZCMat{Gp, n, (long long)n * w},
+                          ZCMat{Tm, w, (long long)n * w}, zmake(1, 0),
+                          ZCMat{M.p, M.ld, M.stride}, M, batch);
+    if (e) return e;
+  }
+  hipLaunchKernelGGL(gj_unswap_kernel, dim3(batch), dim3(256), 0, s, n, M, perm);
+  return (int)hipGetLastError();
+}
+
+int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag) {
+  hipLaunchKernelGGL(identity_kernel, grid_for((long long)n * n, batch), dim3(256), 0, s, n, M,
+                     diag);
+  return (int)hipGetLastError();
+}
+
+int zcopy_batched(hipStream_t s, int rows, int cols, ZCMat src, ZMat dst, int batch) {
+  hipLaunchKernelGGL(copy_kernel, grid_for((long long)rows * cols, batch), dim3(256), 0, s, rows,
+                     cols, src, dst);
+  return (int)hipGetLastError();
+}
+
+int zscale_rc(hipStream_t s, int rows, int cols, ZMat M, const zc* r, long long rstride,
+              const zc* c, long long cstride, int batch) {
+  hipLaunchKernelGGL(scale_rc_kernel, grid_for((long long)rows * cols, batch), dim3(256), 0, s,
+                     rows, cols, M, r, rstride, c, cstride);
+  return (int)hipGetLastError();
+}
+
+int zscale_cols_to(hipStream_t s, int rows, int cols, ZCMat src, const zc* c, long long cstride,
+                   ZMat dst, int batch) {
+  hipLaunchKernelGGL(scale_cols_to_kernel, grid_for((long long)rows * cols, batch), dim3(256), 0,
+                     s, rows, cols, src, c, cstride, dst);
+  return (int)hipGetLastError();
+}
+
+int znormalize_cols(hipStream_t s, int n, ZMat M, int batch) {
+  hipLaunchKernelGGL(normalize_cols_kernel, dim3(n, batch), dim3(256), 0, s, n, M);
+  return (int)hipGetLastError();
+}
+
+int zcount_nonfinite(hipStream_t s, const zc* p, long long count, int* flag) {
+  hipLaunchKernelGGL(nonfinite_kernel, grid_for(count, 1), dim3(256), 0, s, p, count, flag);
+  return (int)hipGetLastError();
+}

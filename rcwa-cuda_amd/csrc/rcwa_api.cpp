@@ -1,0 +1,658 @@
+// Host runtime of the MI355X RCWA solve path and the C ABI of
+// include/rcwa_amd.h: contexts, the HBM arena, the batched layer pipeline,
+// the batched Redheffer fold and the stage entry points.
+//
+// A solve runs the reference's sequence (rcwa3d/src/main.cpp:126-191) with
+// every O(N^3) stage on the GPU, batched over all layers of all solves in a
+// chunk:
+//   ConvMat (truncated DFT + Toeplitz gather) for eps and mu of every layer;
+//   SMatrixLayer for every layer: [[eps]]^-1, [[mu]]^-1, P, Q, Omega^2 = PQ,
+//     eigendecomposition, V, W^-1, V^-1, A, B, A^-1 and the S11/S12 blocks;
+//   the left fold S_dev = ((S_0 * S_1) * ...) per solve, batched over solves;
+//   S_glob = (S_ref * S_dev) * S_trn;
+//   the S11/S21 columns at the two source entries go back to the host, which
+//   forms fields and efficiencies (O(N)).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rcwa_amd.h"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "host_math.hpp"
+#include "runtime.hpp"
+
+using rcwa_host::cplx;
+
+// ------------------------------------------------------------------ errors
+namespace {
+thread_local std::string g_last_error;
+}
+
+rcwa_status rcwa_set_error(rcwa_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+int rcwa_internal_hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  g_last_error = std::string("HIP error ") + hipGetErrorString(e) + " in " + what + " (" + file +
+                 ":" + std::to_string(line) + ")";
+  return RCWA_E_HIP;
+}
+
+#define CK(expr)                                                                   \
+  do {                                                                             \
+    int _e = (int)(expr);                                                          \
+    if (_e) {                                                                      \
+      if (_e != RCWA_E_HIP && _e < 100) return (rcwa_status)_e;                    \
+      return (rcwa_status)rcwa_internal_hip_fail((hipError_t)_e, #expr, __FILE__, __LINE__); \
+    }                                                                              \
+  } while (0)
+#define CKH(expr)                                                                  \
+  do {                                                                             \
+    hipError_t _h = (expr);                                                        \
+    if (_h != hipSuccess)                                                          \
+      return (rcwa_status)rcwa_internal_hip_fail(_h, #expr, __FILE__, __LINE__);   \
+  } while (0)
+
+// ------------------------------------------------------------------ context
+rcwa_status Ctx::init(int dev, size_t bud) {
+  device = dev;
+  CKH(hipSetDevice(dev));
+  CKH(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  size_t fr = 0, tot = 0;
+  CKH(hipMemGetInfo(&fr, &tot));
+  budget = bud ? bud : (size_t)(0.8 * (double)fr);
+  for (auto& e : ev) CKH(hipEventCreate(&e));
+  return RCWA_OK;
+}
+
+void Ctx::release() {
+  if (arena) (void)hipFree(arena);
+  arena = nullptr;
+  arena_cap = 0;
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+rcwa_status Ctx::reserve(size_t bytes) {
+  bytes = (bytes + 4095) & ~(size_t)4095;
+  if (bytes > budget)
+    return rcwa_set_error(RCWA_E_NOMEM, "request needs " + std::to_string(bytes >> 20) +
+                                            " MiB of HBM, budget is " +
+                                            std::to_string(budget >> 20) + " MiB");
+  if (arena_cap < bytes) {
+    CKH(hipSetDevice(device));
+    CKH(hipStreamSynchronize(stream));
+    if (arena) CKH(hipFree(arena));
+    arena = nullptr;
+    arena_cap = 0;
+    CKH(hipMalloc((void**)&arena, bytes));
+    arena_cap = bytes;
+  }
+  arena_off = 0;
+  return RCWA_OK;
+}
+
+void* Ctx::take(size_t bytes) {
+  arena_off = (arena_off + 255) & ~(size_t)255;
+  void* p = arena + arena_off;
+  arena_off += bytes;
+  return p;
+}
+
+// ------------------------------------------------------------------ GEMM dispatch
+int Ctx::gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C,
+              int batch) {
+  if (precision == RCWA_SPLIT)
+    return zgemm_split_batched(stream, M, N, K, alpha, A, B, beta, D, C, batch);
+  return zgemm_batched(stream, M, N, K, alpha, A, B, beta, D, C, batch);
+}
+
+// ------------------------------------------------------------------ layer stage
+size_t layer_work_bytes(int N, int Bl) {
+  const size_t n = 2 * (size_t)N, n2 = n * n;
+  size_t z = (size_t)Bl * (8 * n2 + 2 * (size_t)N * N);
+  return z * sizeof(zc) + gj_workspace_bytes((int)n, Bl) + eig_workspace_bytes((int)n, 0) + 4096;
+}
+
+rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
+                        const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
+                        ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing) {
+  const int n = 2 * N;
+  const long long n2 = (long long)n * n, N2 = (long long)N * N;
+  hipStream_t s = c.stream;
+  auto M = [&](zc* p) { return ZMat{p, n, n2}; };
+  auto CM = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const ZCMat none{nullptr, 0, 0};
+  zc *b0 = w.b[0], *b1 = w.b[1], *b2 = w.b[2], *b3 = w.b[3], *b4 = w.b[4], *b5 = w.b[5];
+  // [[eps]]^-1 and [[mu]]^-1 (rcwa.cpp:267-281: two LU solves each)
+  CK(zcopy_batched(s, N, N, erc, ZMat{w.erci, N, N2}, Bl));
+  CK(zcopy_batched(s, N, N, urc, ZMat{w.urci, N, N2}, Bl));
+  CK(gj_inverse_batched(s, N, ZMat{w.erci, N, N2}, Bl, w.gj, info_sing));
+  CK(gj_inverse_batched(s, N, ZMat{w.urci, N, N2}, Bl, w.gj, info_sing));
+  // P, Q (rcwa.cpp:264-285); Omega^2 = P Q (rcwa.cpp:288)
+  CK(build_pq(s, N, erc, urc, ZCMat{w.erci, N, N2}, ZCMat{w.urci, N, N2}, Kx, Ky, M(b0), M(b1),
+              Bl));
+  CK(c.gemm(n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
+  // eigendecomposition (rcwa.cpp:298-306)
+  if (c.ev_on) CKH(hipEventRecord(c.ev[2], s));
+  CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit));
+  if (c.ev_on) CKH(hipEventRecord(c.ev[3], s));
+  // lambda, X (rcwa.cpp:317, 339); V = Q W diag(1/lambda) (rcwa.cpp:326-330)
+  CK(layer_modes(s, n, lam2, lam, ilam, X, k0t, Bl));
+  CK(zscale_cols_to(s, n, n, CM(b3), ilam, n, M(b0), Bl));
+  CK(c.gemm(n, n, n, one, CM(b1), CM(b0), zero, none, M(b2), Bl));
+  // W^-1 W0 = W^-1, V^-1 V0; A, B (rcwa.cpp:335-338)
+  CK(gj_inverse_batched(s, n, M(b3), Bl, w.gj, info_sing));
+  CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));
+  CK(build_ab(s, N, CM(b3), CM(b2), v0, M(b0), M(b1), Bl));
+  // A^-1
+  CK(zcopy_batched(s, n, n, CM(b0), M(b2), Bl));
+  CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));
+  // M1 = X B A^-1 X
+  CK(c.gemm(n, n, n, one, CM(b1), CM(b2), zero, none, M(b3), Bl));
+  CK(zscale_rc(s, n, n, M(b3), X, n, X, n, Bl));
+  // D = A - M1 B ; R11 = M1 A - B (rcwa.cpp:346-347)
+  CK(c.gemm(n, n, n, mone, CM(b3), CM(b1), one, CM(b0), M(b4), Bl));
+  CK(c.gemm(n, n, n, one, CM(b3), CM(b0), mone, CM(b1), M(b5), Bl));
+  // C2 = A - B A^-1 B (rcwa.cpp:348)
+  CK(c.gemm(n, n, n, one, CM(b2), CM(b1), zero, none, M(b3), Bl));
+  CK(c.gemm(n, n, n, mone, CM(b1), CM(b3), one, CM(b0), M(b2), Bl));
+  // S11 = D^-1 R11 ; S12 = D^-1 X C2
+  CK(gj_inverse_batched(s, n, M(b4), Bl, w.gj, info_sing));
+  CK(c.gemm(n, n, n, one, CM(b4), CM(b5), zero, none, S11, Bl));
+  CK(zscale_rc(s, n, n, M(b4), nullptr, 0, X, n, Bl));
+  CK(c.gemm(n, n, n, one, CM(b4), CM(b2), zero, none, S12, Bl));
+  return RCWA_OK;
+}
+
+void LayerWork::carve(Ctx& c, int N, int Bl) {
+  const size_t n = 2 * (size_t)N, n2 = n * n;
+  for (int i = 0; i < 6; ++i) b[i] = (zc*)c.take((size_t)Bl * n2 * sizeof(zc));
+  erci = (zc*)c.take((size_t)Bl * N * N * sizeof(zc));
+  urci = (zc*)c.take((size_t)Bl * N * N * sizeof(zc));
+  gj = c.take(gj_workspace_bytes((int)n, Bl));
+  eig = c.take(eig_workspace_bytes((int)n, Bl));
+}
+
+// ------------------------------------------------------------------ star product
+// S = A * B (rcwa.cpp:588-615) for a batch.  A's four slots receive S; the
+// slot pointers in A and tmp may be permuted (no copies).
+rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
+                         void* gjws, int* info_sing) {
+  hipStream_t s = c.stream;
+  const long long n2 = (long long)n * n;
+  auto M = [&](zc* p) { return ZMat{p, n, n2}; };
+  auto CM = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const ZCMat none{nullptr, 0, 0};
+  zc *&A11 = A[0], *&A12 = A[1], *&A21 = A[2], *&A22 = A[3];
+  const ZCMat &B11 = B[0], &B12 = B[1], &B21 = B[2], &B22 = B[3];
+  // F1 = I - B11 A22, F2 = I - A22 B11 (rcwa.cpp:602-606), inverted in place
+  CK(zset_identity(s, n, M(tmp[0]), batch, one));
+  CK(c.gemm(n, n, n, mone, B11, CM(A22), one, CM(tmp[0]), M(tmp[0]), batch));
+  CK(zset_identity(s, n, M(tmp[1]), batch, one));
+  CK(c.gemm(n, n, n, mone, CM(A22), B11, one, CM(tmp[1]), M(tmp[1]), batch));
+  CK(gj_inverse_batched(s, n, M(tmp[0]), batch, gjws, info_sing));
+  CK(gj_inverse_batched(s, n, M(tmp[1]), batch, gjws, info_sing));
+  // S11 = A11 + A12 (F1^-1 B11) A21 (rcwa.cpp:608)
+  CK(c.gemm(n, n, n, one, CM(tmp[0]), B11, zero, none, M(tmp[2]), batch));
+  CK(c.gemm(n, n, n, one, CM(A12), CM(tmp[2]), zero, none, M(tmp[3]), batch));
+  CK(c.gemm(n, n, n, one, CM(tmp[3]), CM(A21), one, CM(A11), M(A11), batch));
+  // S12 = A12 (F1^-1 B12) (rcwa.cpp:609)
+  CK(c.gemm(n, n, n, one, CM(tmp[0]), B12, zero, none, M(tmp[2]), batch));
+  CK(c.gemm(n, n, n, one, CM(A12), CM(tmp[2]), zero, none, M(tmp[3]), batch));
+  std::swap(A12, tmp[3]);
+  // S21 = B21 (F2^-1 A21) (rcwa.cpp:610)
+  CK(c.gemm(n, n, n, one, CM(tmp[1]), CM(A21), zero, none, M(tmp[2]), batch));
+  CK(c.gemm(n, n, n, one, B21, CM(tmp[2]), zero, none, M(tmp[4]), batch));
+  std::swap(A21, tmp[4]);
+  // S22 = B22 + B21 (F2^-1 A22) B12 (rcwa.cpp:611)
+  CK(c.gemm(n, n, n, one, CM(tmp[1]), CM(A22), zero, none, M(tmp[2]), batch));
+  CK(c.gemm(n, n, n, one, B21, CM(tmp[2]), zero, none, M(tmp[4]), batch));
+  CK(c.gemm(n, n, n, one, CM(tmp[4]), B12, one, B22, M(A22), batch));
+  return RCWA_OK;
+}
+
+// ------------------------------------------------------------------ solve pipeline
+namespace {
+
+struct Job {
+  const rcwa_device* dev;
+  const rcwa_params* par;
+  rcwa_host::Incidence inc;
+  rcwa_host::Media media;
+  std::vector<int> srcs;   // source indices within the problem
+  const rcwa_source* src;  // problem's source array
+  rcwa_results* out;       // problem's output array
+  rcwa_host::WaveVectors wv;
+};
+
+rcwa_status check_device(const rcwa_device* d, const rcwa_params* p) {
+  if (!d || !p) return rcwa_set_error(RCWA_E_ARG, "null device or params");
+  if (d->Nx <= 0 || d->Ny <= 0 || d->num_layers < 0)
+    return rcwa_set_error(RCWA_E_ARG, "grid sizes must be positive");
+  if (d->num_layers > 0 && (!d->er || !d->ur || !d->t))
+    return rcwa_set_error(RCWA_E_ARG, "null er/ur/t");
+  if (p->Nx_harmonics < 0 || p->Ny_harmonics < 0)
+    return rcwa_set_error(RCWA_E_ARG, "negative harmonic count");
+  if (!(d->Lx > 0) || !(d->Ly > 0)) return rcwa_set_error(RCWA_E_ARG, "periods must be > 0");
+  return RCWA_OK;
+}
+
+inline cplx C(rcwa_c128 z) { return cplx(z.re, z.im); }
+
+struct ChunkPlan {
+  int G, Bl;
+};
+
+size_t chunk_bytes(int N, int L, int Nx, int Ny, int hx, int hy, bool host_grids, int G, int Bl) {
+  const size_t n = 2 * (size_t)N, n2 = n * n, GL = (size_t)G * std::max(L, 1);
+  size_t z = 0;
+  if (host_grids) z += 2 * GL * Nx * Ny;
+  z += GL * Ny * (4 * hx + 1);                  // dft rows
+  z += GL * (4 * hx + 1) * (4 * hy + 1);        // F
+  z += 2 * GL * N * N;                          // erc, urc
+  z += GL * (2 * N + 4 * N) + 4 * GL * n;       // Kx, Ky, v0, lam2/lam/ilam/X
+  z += 2 * GL * n2;                             // layer S11, S12
+  z += 13 * (size_t)G * n2;                     // fold slots
+  z += (size_t)G * 32 * N + (size_t)G * 4 * n;  // end diagonals, gathered columns
+  size_t bytes = z * sizeof(zc) + GL * sizeof(double) + 4 * GL * sizeof(int);
+  bytes += layer_work_bytes(N, Bl) + gj_workspace_bytes((int)n, std::max(G, Bl)) +
+           eig_workspace_bytes((int)n, Bl) + 64 * 256;
+  return bytes;
+}
+
+}  // namespace
+
+rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_out) {
+  const int G = (int)jobs.size();
+  const rcwa_device* d0 = jobs[0]->dev;
+  const rcwa_params* p0 = jobs[0]->par;
+  const int hx = p0->Nx_harmonics, hy = p0->Ny_harmonics;
+  const int P = 2 * hx + 1, Q = 2 * hy + 1, N = P * Q, n = 2 * N;
+  const int L = d0->num_layers, Nx = d0->Nx, Ny = d0->Ny;
+  const bool host_grids = !d0->grids_on_device;
+  const int GL = G * L;
+  const long long n2 = (long long)n * n, N2 = (long long)N * N;
+  hipStream_t s = c.stream;
+
+  // pick the layer sub-batch that fits the budget
+  int Bl = std::max(GL, 1);
+  while (Bl > 1 && chunk_bytes(N, L, Nx, Ny, hx, hy, host_grids, G, Bl) > c.budget) Bl = (Bl + 1) / 2;
+  rcwa_status st = c.reserve(chunk_bytes(N, L, Nx, Ny, hx, hy, host_grids, G, Bl));
+  if (st) return st;
+
+  // ---- host precompute (O(N) per job)
+  std::vector<cplx> hKx((size_t)GL * N), hKy((size_t)GL * N), hv0((size_t)GL * 4 * N);
+  std::vector<double> hk0t(std::max(GL, 1));
+  std::vector<cplx> hends((size_t)G * 32 * N);
+  for (int g = 0; g < G; ++g) {
+    Job& j = *jobs[g];
+    j.wv = rcwa_host::wave_vectors(j.dev->Lx, j.dev->Ly, j.inc, hx, hy, j.media);
+    std::vector<cplx> v0 = rcwa_host::gap_v0_diagonals(j.wv.Kx, j.wv.Ky);
+    const double k0 = 2.0 * M_PI / j.inc.lambda0;
+    for (int l = 0; l < L; ++l) {
+      std::copy(j.wv.Kx.begin(), j.wv.Kx.end(), hKx.begin() + ((size_t)g * L + l) * N);
+      std::copy(j.wv.Ky.begin(), j.wv.Ky.end(), hKy.begin() + ((size_t)g * L + l) * N);
+      std::copy(v0.begin(), v0.end(), hv0.begin() + ((size_t)g * L + l) * 4 * N);
+      hk0t[(size_t)g * L + l] = k0 * j.dev->t[l];
+    }
+    std::vector<cplx> er = rcwa_host::end_smatrix_diagonals(0, j.wv.Kx, j.wv.Ky, j.wv.Kz_ref,
+                                                           j.media.er_ref, j.media.ur_ref);
+    std::vector<cplx> et = rcwa_host::end_smatrix_diagonals(1, j.wv.Kx, j.wv.Ky, j.wv.Kz_trn,
+                                                           j.media.er_trn, j.media.ur_trn);
+    std::copy(er.begin(), er.end(), hends.begin() + (size_t)g * 32 * N);
+    std::copy(et.begin(), et.end(), hends.begin() + (size_t)g * 32 * N + 16 * N);
+  }
+
+  // ---- device buffers
+  const size_t G1 = std::max(GL, std::max(G, 1));
+  zc* dgr_e = nullptr;
+  zc* dgr_u = nullptr;
+  if (host_grids && L > 0) {
+    dgr_e = (zc*)c.take((size_t)GL * Nx * Ny * sizeof(zc));
+    dgr_u = (zc*)c.take((size_t)GL * Nx * Ny * sizeof(zc));
+  }
+  zc* rowbuf = (zc*)c.take(G1 * Ny * (4 * hx + 1) * sizeof(zc));
+  zc* F = (zc*)c.take(G1 * (4 * hx + 1) * (4 * hy + 1) * sizeof(zc));
+  zc* erc = (zc*)c.take(G1 * N2 * sizeof(zc));
+  zc* urc = (zc*)c.take(G1 * N2 * sizeof(zc));
+  zc* dKx = (zc*)c.take(G1 * N * sizeof(zc));
+  zc* dKy = (zc*)c.take(G1 * N * sizeof(zc));
+  zc* dv0 = (zc*)c.take(G1 * 4 * N * sizeof(zc));
+  double* dk0t = (double*)c.take(G1 * sizeof(double));
+  zc* lam2 = (zc*)c.take(G1 * n * sizeof(zc));
+  zc* lam = (zc*)c.take(G1 * n * sizeof(zc));
+  zc* ilam = (zc*)c.take(G1 * n * sizeof(zc));
+  zc* Xd = (zc*)c.take(G1 * n * sizeof(zc));
+  zc* S11l = (zc*)c.take(G1 * n2 * sizeof(zc));
+  zc* S12l = (zc*)c.take(G1 * n2 * sizeof(zc));
+  zc* slot[13];
+  for (int i = 0; i < 13; ++i) slot[i] = (zc*)c.take((size_t)G * n2 * sizeof(zc));
+  zc* dends = (zc*)c.take((size_t)G * 32 * N * sizeof(zc));
+  zc* dcols = (zc*)c.take((size_t)G * 4 * n * sizeof(zc));
+  int* dinfo = (int*)c.take(4 * G1 * sizeof(int));
+  int* dcolidx = (int*)c.take(4 * sizeof(int));
+  void* gjf = c.take(gj_workspace_bytes(n, std::max(G, 1)));
+  LayerWork w;
+  w.carve(c, N, Bl);
+
+  CKH(hipMemsetAsync(dinfo, 0, 4 * G1 * sizeof(int), s));
+  int* info_eig = dinfo;
+  int* info_sing = dinfo + G1;
+  if (L > 0) {
+    CKH(hipMemcpyAsync(dKx, hKx.data(), (size_t)GL * N * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dKy, hKy.data(), (size_t)GL * N * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dv0, hv0.data(), (size_t)GL * 4 * N * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dk0t, hk0t.data(), (size_t)GL * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  CKH(hipMemcpyAsync(dends, hends.data(), (size_t)G * 32 * N * sizeof(zc), hipMemcpyHostToDevice, s));
+  const int hcol[4] = {N / 2, N / 2 + N, 0, 0};
+  CKH(hipMemcpyAsync(dcolidx, hcol, sizeof(hcol), hipMemcpyHostToDevice, s));
+
+  c.ev_on = true;
+  CKH(hipEventRecord(c.ev[0], s));
+  // ---- ConvMat for every layer (main.cpp:126-138)
+  if (L > 0) {
+    const size_t G2 = (size_t)Nx * Ny;
+    for (int g = 0; g < G; ++g) {
+      const rcwa_device* d = jobs[g]->dev;
+      const zc *ge, *gu;
+      if (host_grids) {
+        CKH(hipMemcpyAsync(dgr_e + (size_t)g * L * G2, d->er, (size_t)L * G2 * sizeof(zc),
+                           hipMemcpyHostToDevice, s));
+        CKH(hipMemcpyAsync(dgr_u + (size_t)g * L * G2, d->ur, (size_t)L * G2 * sizeof(zc),
+                           hipMemcpyHostToDevice, s));
+        ge = dgr_e + (size_t)g * L * G2;
+        gu = dgr_u + (size_t)g * L * G2;
+      } else {
+        ge = (const zc*)d->er;
+        gu = (const zc*)d->ur;
+      }
+      CK(convmat_dft(s, ge, Nx, Ny, hx, hy, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), L,
+                     rowbuf + (size_t)g * L * Ny * (4 * hx + 1)));
+      CK(convmat_gather(s, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), Nx, Ny, hx, hy,
+                        ZMat{erc + (size_t)g * L * N2, N, N2}, L));
+      CK(convmat_dft(s, gu, Nx, Ny, hx, hy, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), L,
+                     rowbuf + (size_t)g * L * Ny * (4 * hx + 1)));
+      CK(convmat_gather(s, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), Nx, Ny, hx, hy,
+                        ZMat{urc + (size_t)g * L * N2, N, N2}, L));
+    }
+  }
+  CKH(hipEventRecord(c.ev[1], s));
+  // ---- SMatrixLayer for every layer (main.cpp:151)
+  float ms_eig = 0;
+  for (int l0 = 0; l0 < GL; l0 += Bl) {
+    const int b = std::min(Bl, GL - l0);
+    st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
+                     dKx + (size_t)l0 * N, dKy + (size_t)l0 * N, dv0 + (size_t)l0 * 4 * N,
+                     dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
+                     ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
+                     ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
+                     info_eig + l0, info_sing + l0);
+    if (st) return st;
+    CKH(hipEventSynchronize(c.ev[3]));
+    float t = 0;
+    CKH(hipEventElapsedTime(&t, c.ev[2], c.ev[3]));
+    ms_eig += t;
+  }
+  CKH(hipEventRecord(c.ev[4], s));
+  // ---- fold (main.cpp:143-164), ends (main.cpp:166-170)
+  zc* S[4] = {slot[0], slot[1], slot[2], slot[3]};
+  zc* E[4] = {slot[4], slot[5], slot[6], slot[7]};
+  zc* T[5] = {slot[8], slot[9], slot[10], slot[11], slot[12]};
+  if (L == 0) {
+    // SMatrixInit (rcwa.cpp:237-250)
+    CK(zset_identity(s, n, ZMat{S[0], n, n2}, G, zmake(0, 0)));
+    CK(zset_identity(s, n, ZMat{S[1], n, n2}, G, zmake(1, 0)));
+    CK(zset_identity(s, n, ZMat{S[2], n, n2}, G, zmake(1, 0)));
+    CK(zset_identity(s, n, ZMat{S[3], n, n2}, G, zmake(0, 0)));
+  } else {
+    // SMatrixInit * S_0 = S_0 exactly in IEEE arithmetic (every product is
+    // with I or 0), so the fold starts from S_0.
+    const long long lstride = (long long)L * n2;
+    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, lstride}, ZMat{S[0], n, n2}, G));
+    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, lstride}, ZMat{S[1], n, n2}, G));
+    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, lstride}, ZMat{S[2], n, n2}, G));
+    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, lstride}, ZMat{S[3], n, n2}, G));
+    for (int l = 1; l < L; ++l) {
+      const ZCMat B[4] = {{S11l + (size_t)l * n2, n, lstride}, {S12l + (size_t)l * n2, n, lstride},
+                          {S12l + (size_t)l * n2, n, lstride}, {S11l + (size_t)l * n2, n, lstride}};
+      st = star_product(c, n, G, S, B, T, gjf, info_sing);
+      if (st) return st;
+    }
+  }
+  // per-job end diagonals: [job][ref S11..S22, trn S11..S22][4][N]
+  for (int k = 0; k < 4; ++k)
+    CK(expand_block_diag(s, N, dends + (size_t)4 * k * N, 32LL * N, ZMat{E[k], n, n2}, G));
+  {
+    const ZCMat B[4] = {{S[0], n, n2}, {S[1], n, n2}, {S[2], n, n2}, {S[3], n, n2}};
+    st = star_product(c, n, G, E, B, T, gjf, info_sing);
+    if (st) return st;
+  }
+  for (int k = 0; k < 4; ++k)
+    CK(expand_block_diag(s, N, dends + 16 * (size_t)N + (size_t)4 * k * N, 32LL * N,
+                         ZMat{S[k], n, n2}, G));
+  {
+    const ZCMat B[4] = {{S[0], n, n2}, {S[1], n, n2}, {S[2], n, n2}, {S[3], n, n2}};
+    st = star_product(c, n, G, E, B, T, gjf, info_sing);
+    if (st) return st;
+  }
+  CKH(hipEventRecord(c.ev[5], s));
+  // ---- S11 / S21 columns at the source entries (rcwa.cpp:512-514, 535, 553)
+  CK(gather_cols(s, n, ZCMat{E[0], n, n2}, dcolidx, 2, dcols, G));
+  CK(gather_cols(s, n, ZCMat{E[2], n, n2}, dcolidx, 2, dcols + (size_t)G * 2 * n, G));
+  std::vector<cplx> hcols((size_t)G * 4 * n);
+  CKH(hipMemcpyAsync(hcols.data(), dcols, hcols.size() * sizeof(zc), hipMemcpyDeviceToHost, s));
+  std::vector<int> hinfo(2 * G1);
+  CKH(hipMemcpyAsync(hinfo.data(), dinfo, 2 * G1 * sizeof(int), hipMemcpyDeviceToHost, s));
+  CKH(hipEventRecord(c.ev[6], s));
+  CKH(hipStreamSynchronize(s));
+  c.ev_on = false;
+  float t01 = 0, t14 = 0, t45 = 0, t56 = 0;
+  CKH(hipEventElapsedTime(&t01, c.ev[0], c.ev[1]));
+  CKH(hipEventElapsedTime(&t14, c.ev[1], c.ev[4]));
+  CKH(hipEventElapsedTime(&t45, c.ev[4], c.ev[5]));
+  CKH(hipEventElapsedTime(&t56, c.ev[5], c.ev[6]));
+  c.stats.ms_convmat += t01;
+  c.stats.ms_layers += t14;
+  c.stats.ms_eig += ms_eig;
+  c.stats.ms_fold += t45;
+  c.stats.ms_out += t56;
+  c.stats.n_smatrices += G;
+  c.stats.n_layers += GL;
+  const double N3 = (double)N * N * N;
+  c.stats.flops_algorithmic += (2384.0 * GL + 1621.0 * G) * N3;
+
+  for (int g = 0; g < GL; ++g)
+    if (hinfo[g]) return rcwa_set_error(RCWA_E_EIG_NOCONV, "eigensolver QR did not converge");
+  for (size_t g = 0; g < G1; ++g)
+    if (hinfo[G1 + g]) return rcwa_set_error(RCWA_E_SINGULAR, "exactly singular matrix inverted");
+
+  // ---- efficiencies per source (host, O(N))
+  for (int g = 0; g < G; ++g) {
+    Job& j = *jobs[g];
+    const cplx* s11c = hcols.data() + (size_t)g * 2 * n;
+    const cplx* s21c = hcols.data() + (size_t)G * 2 * n + (size_t)g * 2 * n;
+    if (cols_out) {
+      cols_out->insert(cols_out->end(), s11c, s11c + 2 * n);
+      cols_out->insert(cols_out->end(), s21c, s21c + 2 * n);
+    }
+    for (int si : j.srcs) {
+      const rcwa_source& src = j.src[si];
+      rcwa_host::Efficiencies e =
+          rcwa_host::efficiencies(src.theta, src.pte, src.ptm, j.wv, j.media, s11c, s21c);
+      rcwa_results& o = j.out[si];
+      if (o.R) std::memcpy(o.R, e.R.data(), N * sizeof(double));
+      if (o.T) std::memcpy(o.T, e.T.data(), N * sizeof(double));
+      o.R_tot = e.R_tot;
+      o.T_tot = e.T_tot;
+      if (!std::isfinite(e.R_tot) || !std::isfinite(e.T_tot))
+        return rcwa_set_error(RCWA_E_NONFINITE, "non-finite efficiencies");
+    }
+  }
+  return RCWA_OK;
+}
+
+rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_precision prec) {
+  c.precision = prec;
+  c.last_cols.clear();
+  std::vector<Job> jobs;
+  for (int p = 0; p < nprob; ++p) {
+    const rcwa_problem& pr = probs[p].p;
+    rcwa_status st = check_device(pr.device, pr.params);
+    if (st) return st;
+    if (pr.nsrc <= 0 || !pr.src) return rcwa_set_error(RCWA_E_ARG, "problem without sources");
+    std::vector<size_t> mine;
+    for (int si = 0; si < pr.nsrc; ++si) {
+      const rcwa_source& sr = pr.src[si];
+      if (!(sr.lambda0 > 0)) return rcwa_set_error(RCWA_E_ARG, "lambda0 must be > 0");
+      bool found = false;
+      for (size_t k : mine) {
+        Job& j = jobs[k];
+        if (j.inc.lambda0 == sr.lambda0 && j.inc.theta == sr.theta && j.inc.phi == sr.phi) {
+          j.srcs.push_back(si);
+          found = true;
+          break;
+        }
+      }
+      if (!found) {
+        Job j;
+        j.dev = pr.device;
+        j.par = pr.params;
+        j.inc = {sr.lambda0, sr.theta, sr.phi};
+        j.media = {C(pr.params->er_ref), C(pr.params->ur_ref), C(pr.params->er_trn),
+                   C(pr.params->ur_trn)};
+        j.srcs.push_back(si);
+        j.src = pr.src;
+        j.out = probs[p].out;
+        mine.push_back(jobs.size());
+        jobs.push_back(j);
+      }
+    }
+  }
+  // group by shape
+  std::map<std::tuple<int, int, int, int, int, int>, std::vector<Job*>> groups;
+  for (auto& j : jobs)
+    groups[{j.par->Nx_harmonics, j.par->Ny_harmonics, j.dev->num_layers, j.dev->Nx, j.dev->Ny,
+            j.dev->grids_on_device}]
+        .push_back(&j);
+  for (auto& kv : groups) {
+    std::vector<Job*>& v = kv.second;
+    const Job& j0 = *v[0];
+    const int hx = j0.par->Nx_harmonics, hy = j0.par->Ny_harmonics;
+    const int N = (2 * hx + 1) * (2 * hy + 1);
+    int G = (int)v.size();
+    if (c.max_chunk > 0) G = std::min(G, c.max_chunk);
+    while (G > 1 && chunk_bytes(N, j0.dev->num_layers, j0.dev->Nx, j0.dev->Ny, hx, hy,
+                                !j0.dev->grids_on_device, G, 1) > c.budget)
+      G = (G + 1) / 2;
+    for (size_t i = 0; i < v.size(); i += G) {
+      std::vector<Job*> part(v.begin() + i, v.begin() + std::min(v.size(), i + (size_t)G));
+      rcwa_status st = run_chunk(c, part, &c.last_cols);
+      if (st) return st;
+    }
+  }
+  return RCWA_OK;
+}
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* rcwa_last_error(void) { return g_last_error.c_str(); }
+const char* rcwa_version(void) { return "rcwa_amd 0.1 (gfx950)"; }
+
+rcwa_status rcwa_ctx_create(int device, size_t hbm_budget, rcwa_ctx** out) {
+  if (!out) return rcwa_set_error(RCWA_E_ARG, "null out");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return rcwa_set_error(RCWA_E_HIP, "no HIP device available");
+  if (device < 0 || device >= count) return rcwa_set_error(RCWA_E_ARG, "bad device index");
+  Ctx* c = new Ctx();
+  rcwa_status st = c->init(device, hbm_budget);
+  if (st) {
+    c->release();
+    delete c;
+    return st;
+  }
+  *out = reinterpret_cast<rcwa_ctx*>(c);
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_destroy(rcwa_ctx* ctx) {
+  if (!ctx) return RCWA_OK;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->release();
+  delete c;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_set_eig_maxit(rcwa_ctx* ctx, int maxit) {
+  if (!ctx || maxit < 0) return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  reinterpret_cast<Ctx*>(ctx)->eig_maxit = maxit;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_set_chunk(rcwa_ctx* ctx, int max_chunk) {
+  if (!ctx) return rcwa_set_error(RCWA_E_ARG, "null ctx");
+  reinterpret_cast<Ctx*>(ctx)->max_chunk = max_chunk;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_get_stats(const rcwa_ctx* ctx, rcwa_stats* out) {
+  if (!ctx || !out) return rcwa_set_error(RCWA_E_ARG, "null argument");
+  *out = reinterpret_cast<const Ctx*>(ctx)->stats;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_reset_stats(rcwa_ctx* ctx) {
+  if (!ctx) return rcwa_set_error(RCWA_E_ARG, "null ctx");
+  reinterpret_cast<Ctx*>(ctx)->stats = rcwa_stats{};
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_solve(rcwa_ctx* ctx, const rcwa_device* dev, const rcwa_params* params,
+                       const rcwa_source* src, int nsrc, rcwa_precision precision,
+                       rcwa_results* out) {
+  rcwa_problem p{dev, params, src, nsrc};
+  return rcwa_solve_batch(ctx, &p, 1, precision, out);
+}
+
+rcwa_status rcwa_solve_batch(rcwa_ctx* ctx, const rcwa_problem* probs, int nprob,
+                             rcwa_precision precision, rcwa_results* out) {
+  if (!ctx || !probs || nprob <= 0 || !out) return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  if (precision != RCWA_FP64 && precision != RCWA_SPLIT)
+    return rcwa_set_error(RCWA_E_ARG, "unknown precision");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  CKH(hipSetDevice(c->device));
+  std::vector<ProblemRef> ps(nprob);
+  size_t off = 0;
+  for (int i = 0; i < nprob; ++i) {
+    ps[i].p = probs[i];
+    ps[i].out = out + off;
+    off += probs[i].nsrc;
+  }
+  return solve_problems(*c, ps.data(), nprob, precision);
+}
+
+rcwa_status rcwa_last_source_columns(rcwa_ctx* ctx, rcwa_c128* cols, size_t count) {
+  if (!ctx || !cols) return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (count > c->last_cols.size()) return rcwa_set_error(RCWA_E_ARG, "count too large");
+  std::memcpy((void*)cols, c->last_cols.data(), count * sizeof(rcwa_c128));
+  return RCWA_OK;
+}
+
+}  // extern "C"

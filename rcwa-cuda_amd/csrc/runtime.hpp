@@ -1,0 +1,64 @@
+// Internal host runtime types shared by the solve path and the server.
+#pragma once
+#include <string>
+#include <vector>
+#include <complex>
+
+#include "rcwa_amd.h"
+#include "common.hpp"
+
+rcwa_status rcwa_set_error(rcwa_status s, const std::string& msg);
+
+// Split-precision complex GEMM (FP32 matrix cores, FP32 accumulation);
+// defined in split_gemm.hip.
+int zgemm_split_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                        ZCMat D, ZMat C, int batch);
+
+// A solve context: one HIP stream, an HBM arena bounded by `budget`.
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_off = 0;
+  int eig_maxit = 30;
+  int max_chunk = 0;
+  rcwa_precision precision = RCWA_FP64;
+  rcwa_stats stats{};
+  std::vector<std::complex<double>> last_cols;
+  hipEvent_t ev[8] = {};
+  bool ev_on = false;
+
+  rcwa_status init(int dev, size_t budget);
+  void release();
+  rcwa_status reserve(size_t bytes);  // arena of at least `bytes`; resets the bump pointer
+  void* take(size_t bytes);
+  int gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch);
+};
+
+// Work buffers of the layer stage for a sub-batch of Bl layers.
+struct LayerWork {
+  zc* b[6];
+  zc* erci;
+  zc* urci;
+  void* gj;
+  void* eig;
+  void carve(Ctx& c, int N, int Bl);
+};
+
+size_t layer_work_bytes(int N, int Bl);
+
+rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
+                        const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
+                        ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing);
+
+rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
+                         void* gjws, int* info_sing);
+
+// A problem with the place its results go.
+struct ProblemRef {
+  rcwa_problem p;
+  rcwa_results* out;
+};
+
+rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_precision prec);

@@ -1,0 +1,141 @@
+// Batched complex128 GEMM on the gfx950 FP64 matrix cores.
+//
+//   C[b] = alpha * A[b] * B[b] + beta * D[b]      (column-major)
+//
+// Every dense product of the path goes through here: Omega^2 = P Q
+// (rcwa.cpp:288), V = Q W (rcwa.cpp:326), the layer S-matrix products
+// (rcwa.cpp:345-348), the Redheffer products (rcwa.cpp:602-611), the
+// eigenvector back-transformation and the rank-nb updates of the blocked
+// Gauss-Jordan inverse.
+//
+// A complex product is four real FP64 MFMAs per 16x16x4 step on the same
+// operand registers: each lane loads its complex element once and feeds the
+// real part to two MFMAs and the imaginary part to the other two
+// (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), so no deinterleaving is needed.
+//
+// Tiling: 256 threads = 4 wave64s in a 2x2 arrangement; the workgroup tile
+// is 64x64 of C, each wave owns 32x32 = 2x2 MFMA tiles of 16x16.  K advances
+// in slabs of 16 staged through LDS (double-buffered: the next slab's global
+// loads are issued before the MFMAs on the current one).
+#include "common.hpp"
+#include "kernels.hpp"
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int PADA = 1, PADB = 1;
+
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alpha, ZCMat A,
+                                                    ZCMat B, zc beta, ZCMat D, ZMat C,
+                                                    int tiles_n) {
+  __shared__ zc As[2][BK][BM + PADA];
+  __shared__ zc Bs[2][BK][BN + PADB];
+  const int b = blockIdx.z;
+  const int tn = blockIdx.x % tiles_n;
+  const int tm = blockIdx.x / tiles_n;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const zc* Ap = A.p + (long long)b * A.stride;
+  const zc* Bp = B.p + (long long)b * B.stride;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, lk = lane >> 4;
+
+  v4d accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = v4d{0, 0, 0, 0};
+      accI[i][j] = v4d{0, 0, 0, 0};
+    }
+
+  zc ra[4], rb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + 256 * r;
+      const int i = idx & (BM - 1), k = idx >> 6;       // A: rows fastest
+      const int gi = m0 + i, gk = k0 + k;
+      ra[r] = (gi < M && gk < K) ? Ap[gi + (long long)gk * A.ld] : zmake(0, 0);
+      const int kb = idx & (BK - 1), j = idx >> 4;        // B: k fastest
+      const int gkb = k0 + kb, gj = n0 + j;
+      rb[r] = (gkb < K && gj < N) ? Bp[gkb + (long long)gj * B.ld] : zmake(0, 0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = tid + 256 * r;
+      As[buf][idx >> 6][idx & (BM - 1)] = ra[r];
+      Bs[buf][idx & (BK - 1)][idx >> 4] = rb[r];
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int t = 0; t < nk; ++t) {
+    const int cur = t & 1;
+    if (t + 1 < nk) gload((t + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      zc a[2], bb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[cur][kk + lk][wm * 32 + i * 16 + li];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bb[j] = Bs[cur][kk + lk][wn * 32 + j * 16 + li];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = mfma(a[i].x, bb[j].x, accR[i][j]);
+          accR[i][j] = mfma(-a[i].y, bb[j].y, accR[i][j]);
+          accI[i][j] = mfma(a[i].x, bb[j].y, accI[i][j]);
+          accI[i][j] = mfma(a[i].y, bb[j].x, accI[i][j]);
+        }
+    }
+    if (t + 1 < nk) {
+      sstore(cur ^ 1);
+    }
+    __syncthreads();
+  }
+
+  zc* Cp = C.p + (long long)b * C.stride;
+  const zc* Dp = D.p ? D.p + (long long)b * D.stride : nullptr;
+  const bool use_d = (beta.x != 0.0 || beta.y != 0.0) && Dp != nullptr;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn * 32 + j * 16 + li;
+      if (col >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 32 + i * 16 + 4 * lk + r;
+        if (row >= M) continue;
+        zc v = zmul(alpha, zmake(accR[i][j][r], accI[i][j][r]));
+        if (use_d) v = zfma(beta, Dp[row + (long long)col * D.ld], v);
+        Cp[row + (long long)col * C.ld] = v;
+      }
+    }
+}
+
+}  // namespace
+
+int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                  ZCMat D, ZMat C, int batch) {
+  if (M <= 0 || N <= 0 || batch <= 0) return 0;
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  dim3 grid(tiles_m * tiles_n, 1, batch);
+  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, s, M, N, K, alpha, A, B, beta, D, C,
+                     tiles_n);
+  return (int)hipGetLastError();
+}

@@ -1,0 +1,317 @@
+"""ctypes binding of librcwa_amd.so (the C ABI declared in include/rcwa_amd.h).
+
+The library is the product: every call below runs the HIP kernels of
+csrc/ on the GPU.  There is no fallback: if the shared library is not built,
+or no GPU is present, calls raise.
+"""
+import ctypes
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librcwa_amd.so")
+
+RCWA_FP64 = 0
+RCWA_SPLIT = 1
+STATUS = {0: "OK", 1: "E_ARG", 2: "E_NOMEM", 3: "E_EIG_NOCONV", 4: "E_SINGULAR",
+          5: "E_NONFINITE", 6: "E_HIP", 7: "E_NOTREADY", 8: "E_CANCELLED"}
+
+
+class RcwaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (STATUS.get(code, code), msg))
+        self.code = code
+
+
+class c128(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+def _c(z):
+    z = complex(z)
+    return c128(z.real, z.imag)
+
+
+class Device(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("num_layers", ctypes.c_int),
+                ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+                ("er", ctypes.c_void_p), ("ur", ctypes.c_void_p), ("t", ctypes.c_void_p),
+                ("grids_on_device", ctypes.c_int)]
+
+
+class Source(ctypes.Structure):
+    _fields_ = [("lambda0", ctypes.c_double), ("theta", ctypes.c_double),
+                ("phi", ctypes.c_double), ("pte", ctypes.c_double), ("ptm", ctypes.c_double)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("Nx_harmonics", ctypes.c_int), ("Ny_harmonics", ctypes.c_int),
+                ("er_ref", c128), ("ur_ref", c128), ("er_trn", c128), ("ur_trn", c128)]
+
+
+class Results(ctypes.Structure):
+    _fields_ = [("R", ctypes.c_void_p), ("T", ctypes.c_void_p),
+                ("R_tot", ctypes.c_double), ("T_tot", ctypes.c_double)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("device", ctypes.POINTER(Device)), ("params", ctypes.POINTER(Params)),
+                ("src", ctypes.POINTER(Source)), ("nsrc", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_convmat", ctypes.c_double),
+                ("ms_layers", ctypes.c_double), ("ms_eig", ctypes.c_double),
+                ("ms_fold", ctypes.c_double), ("ms_out", ctypes.c_double),
+                ("flops_algorithmic", ctypes.c_double), ("n_smatrices", ctypes.c_int),
+                ("n_layers", ctypes.c_int)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch", ctypes.c_int),
+                ("n_workers", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class TicketInfo(ctypes.Structure):
+    _fields_ = [("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_finish", ctypes.c_double), ("batch_size", ctypes.c_int)]
+
+
+# Every function declared in include/rcwa_amd.h.
+EXPORTS = [
+    "rcwa_last_error", "rcwa_version", "rcwa_ctx_create", "rcwa_ctx_destroy",
+    "rcwa_ctx_set_eig_maxit", "rcwa_ctx_get_stats", "rcwa_ctx_reset_stats", "rcwa_ctx_set_chunk",
+    "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
+    "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
+    "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
+    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_server_create", "rcwa_server_submit",
+    "rcwa_server_poll", "rcwa_server_wait", "rcwa_server_cancel", "rcwa_server_clock",
+    "rcwa_server_destroy",
+]
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librcwa_amd.so is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.rcwa_last_error.restype = ctypes.c_char_p
+        L.rcwa_version.restype = ctypes.c_char_p
+        L.rcwa_server_clock.restype = ctypes.c_double
+        L.rcwa_server_clock.argtypes = [ctypes.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != 0:
+        raise RcwaError(rc, lib().rcwa_last_error().decode())
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _cz(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a if shape is None else a.reshape(shape)
+
+
+def _fz(a):
+    """Column-major complex128 array (the library's matrix layout)."""
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+def _stack_f(mats):
+    """Batch of matrices -> contiguous [b][col-major n*m] buffer."""
+    mats = [np.asarray(m, np.complex128) for m in mats]
+    return np.ascontiguousarray(np.stack([m.T for m in mats]))  # m.T row-major == m col-major
+
+
+def _unstack_f(buf, b, r, c):
+    return np.stack([buf[i].reshape(c, r).T for i in range(b)])
+
+
+class Context:
+    """A solve context on one GPU (rcwa_ctx)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.h = ctypes.c_void_p()
+        _check(lib().rcwa_ctx_create(device, ctypes.c_size_t(hbm_budget), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().rcwa_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_eig_maxit(self, m):
+        _check(lib().rcwa_ctx_set_eig_maxit(self.h, m))
+
+    def set_chunk(self, m):
+        _check(lib().rcwa_ctx_set_chunk(self.h, m))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().rcwa_ctx_get_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def reset_stats(self):
+        _check(lib().rcwa_ctx_reset_stats(self.h))
+
+    # ---------------------------------------------------------------- solves
+    def solve(self, probs, precision=RCWA_FP64, device_grids=None):
+        """Solve a list of workloads.Problem.  Returns a list (per problem) of
+        (R[npol][Q][P], T[npol][Q][P], tot[npol][2]).  device_grids, if given,
+        is a list of (er_dev_ptr, ur_dev_ptr) for HBM-resident grids."""
+        keep = []
+        cprobs = (Problem * len(probs))()
+        outs = []
+        total_src = sum(len(p.pols) for p in probs)
+        res = (Results * total_src)()
+        k = 0
+        for i, p in enumerate(probs):
+            P, Q = 2 * p.hx + 1, 2 * p.hy + 1
+            npol = len(p.pols)
+            R = np.zeros((npol, P * Q))
+            T = np.zeros((npol, P * Q))
+            outs.append((R, T))
+            if device_grids is not None:
+                er_p, ur_p = device_grids[i]
+                ondev = 1
+            else:
+                er_p, ur_p = p.er.ctypes.data, p.ur.ctypes.data
+                ondev = 0
+            d = Device(p.er.shape[2], p.er.shape[1], p.L, p.Lx, p.Ly, er_p, ur_p,
+                       p.thick.ctypes.data, ondev)
+            par = Params(p.hx, p.hy, _c(p.er_ref), _c(p.ur_ref), _c(p.er_trn), _c(p.ur_trn))
+            srcs = (Source * npol)(*[Source(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols])
+            keep += [d, par, srcs, p]
+            cprobs[i] = Problem(ctypes.pointer(d), ctypes.pointer(par), srcs, npol)
+            for j in range(npol):
+                res[k] = Results(R[j].ctypes.data, T[j].ctypes.data, 0.0, 0.0)
+                k += 1
+        _check(lib().rcwa_solve_batch(self.h, cprobs, len(probs), precision, res))
+        out = []
+        k = 0
+        for i, p in enumerate(probs):
+            P, Q = 2 * p.hx + 1, 2 * p.hy + 1
+            npol = len(p.pols)
+            R, T = outs[i]
+            tot = np.array([[res[k + j].R_tot, res[k + j].T_tot] for j in range(npol)])
+            k += npol
+            out.append((R.reshape(npol, P, Q).transpose(0, 2, 1),
+                        T.reshape(npol, P, Q).transpose(0, 2, 1), tot))
+        return out
+
+    def last_source_columns(self, count):
+        buf = np.zeros(count, np.complex128)
+        _check(lib().rcwa_last_source_columns(self.h, _ptr(buf), ctypes.c_size_t(count)))
+        return buf
+
+    # ---------------------------------------------------------------- stages
+    def convmat(self, grids, layer, hx, hy):
+        g = _cz(grids)
+        L, Ny, Nx = g.shape
+        PQ = (2 * hx + 1) * (2 * hy + 1)
+        out = np.zeros(PQ * PQ, np.complex128)
+        _check(lib().rcwa_convmat(self.h, _ptr(g), layer, Nx, Ny, hx, hy, _ptr(out)))
+        return out.reshape(PQ, PQ).T
+
+    def layer_smatrix(self, ercs, urcs, Kx, Ky, k0t):
+        b = len(ercs)
+        N = ercs[0].shape[0]
+        e = _stack_f(ercs)
+        u = _stack_f(urcs)
+        kx = _cz(Kx)
+        ky = _cz(Ky)
+        kt = np.ascontiguousarray(k0t, np.float64)
+        n = 2 * N
+        S11 = np.zeros((b, n * n), np.complex128)
+        S12 = np.zeros((b, n * n), np.complex128)
+        _check(lib().rcwa_layer_smatrix(self.h, N, b, _ptr(e), _ptr(u), _ptr(kx), _ptr(ky),
+                                        _ptr(kt), _ptr(S11), _ptr(S12)))
+        return _unstack_f(S11, b, n, n), _unstack_f(S12, b, n, n)
+
+    def star(self, A, B):
+        """A, B: tuples of 4 arrays [batch][n][n]."""
+        b, n = A[0].shape[0], A[0].shape[1]
+        As = [_stack_f(list(x)) for x in A]
+        Bs = [_stack_f(list(x)) for x in B]
+        Ss = [np.zeros((b, n * n), np.complex128) for _ in range(4)]
+        arr = ctypes.c_void_p * 4
+        _check(lib().rcwa_star_product(self.h, n, b, arr(*[x.ctypes.data for x in As]),
+                                       arr(*[x.ctypes.data for x in Bs]),
+                                       arr(*[x.ctypes.data for x in Ss])))
+        return tuple(_unstack_f(x, b, n, n) for x in Ss)
+
+    def eig(self, mats):
+        b, n = len(mats), mats[0].shape[0]
+        A = _stack_f(mats)
+        w = np.zeros((b, n), np.complex128)
+        V = np.zeros((b, n * n), np.complex128)
+        _check(lib().rcwa_eig(self.h, n, b, _ptr(A), _ptr(w), _ptr(V)))
+        return w, _unstack_f(V, b, n, n)
+
+    def inverse(self, mats):
+        b, n = len(mats), mats[0].shape[0]
+        A = _stack_f(mats)
+        out = np.zeros((b, n * n), np.complex128)
+        _check(lib().rcwa_inverse(self.h, n, b, _ptr(A), _ptr(out)))
+        return _unstack_f(out, b, n, n)
+
+    def zgemm(self, As, Bs, split=False):
+        b = len(As)
+        m, k = As[0].shape
+        n = Bs[0].shape[1]
+        A = _stack_f(As)
+        B = _stack_f(Bs)
+        C = np.zeros((b, m * n), np.complex128)
+        fn = lib().rcwa_zgemm_split if split else lib().rcwa_zgemm
+        _check(fn(self.h, m, n, k, b, _ptr(A), _ptr(B), _ptr(C)))
+        return _unstack_f(C, b, m, n)
+
+
+# ---------------------------------------------------------------- host stages
+def wave_vectors(Lx, Ly, lam0, theta, phi, hx, hy, er_ref, ur_ref, er_trn, ur_trn):
+    PQ = (2 * hx + 1) * (2 * hy + 1)
+    src = Source(lam0, theta, phi, 1.0, 0.0)
+    par = Params(hx, hy, _c(er_ref), _c(ur_ref), _c(er_trn), _c(ur_trn))
+    k = np.zeros(3, np.complex128)
+    o = [np.zeros(PQ, np.complex128) for _ in range(4)]
+    _check(lib().rcwa_wave_vectors(ctypes.c_double(Lx), ctypes.c_double(Ly), ctypes.byref(src),
+                                   ctypes.byref(par), _ptr(k), *[_ptr(x) for x in o]))
+    return (k, *o)
+
+
+def gap_medium(Kx, Ky):
+    PQ = len(Kx)
+    W0 = np.zeros(4 * PQ * PQ, np.complex128)
+    V0 = np.zeros(4 * PQ * PQ, np.complex128)
+    _check(lib().rcwa_gap_medium(PQ, _ptr(_cz(Kx)), _ptr(_cz(Ky)), _ptr(W0), _ptr(V0)))
+    n = 2 * PQ
+    return W0.reshape(n, n).T, V0.reshape(n, n).T
+
+
+def end_smatrix(which, hx, hy, Kx, Ky, Kz, er_ref, ur_ref, er_trn, ur_trn):
+    PQ = len(Kx)
+    n = 2 * PQ
+    par = Params(hx, hy, _c(er_ref), _c(ur_ref), _c(er_trn), _c(ur_trn))
+    S = [np.zeros(n * n, np.complex128) for _ in range(4)]
+    _check(lib().rcwa_end_smatrix(which, PQ, ctypes.byref(par), _ptr(_cz(Kx)), _ptr(_cz(Ky)),
+                                  _ptr(_cz(Kz)), *[_ptr(x) for x in S]))
+    return tuple(x.reshape(n, n).T for x in S)

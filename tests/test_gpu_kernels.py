@@ -1,0 +1,87 @@
+"""GPU kernels against numpy / the oracle: ZGEMM on the FP64 matrix cores,
+batched Gauss-Jordan inverse, batched eigendecomposition, ConvMat."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import rcwa_amd as G
+
+pytestmark = pytest.mark.gpu
+
+
+def rnd(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (16, 16, 4), (17, 33, 5), (64, 64, 64),
+                                   (100, 70, 130), (242, 242, 242)])
+def test_zgemm_fp64(ctx, m, n, k):
+    rng = np.random.default_rng(m * 1000 + n * 10 + k)
+    A = [rnd(rng, m, k) for _ in range(3)]
+    B = [rnd(rng, k, n) for _ in range(3)]
+    C = ctx.zgemm(A, B)
+    for a, b, c in zip(A, B, C):
+        ref = a @ b
+        err = np.abs(c - ref).max() / (np.abs(a).max() * np.abs(b).max() * k)
+        assert err < 1e-15, err
+
+
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 100, 242])
+def test_inverse(ctx, n):
+    rng = np.random.default_rng(n)
+    A = [rnd(rng, n, n) + 2 * np.sqrt(n) * np.eye(n) for _ in range(4)]
+    X = ctx.inverse(A)
+    for a, x in zip(A, X):
+        err = np.abs(a @ x - np.eye(n)).max()
+        assert err < 1e-12 * n, err
+        assert np.abs(x - np.linalg.inv(a)).max() / np.abs(x).max() < 1e-12
+
+
+def test_inverse_needs_pivoting(ctx):
+    # zero leading entry forces a row swap at the first step
+    A = np.array([[0, 1, 2], [3, 4, 5], [6, 7, 9]], np.complex128) + 0.5j * np.eye(3)
+    X = ctx.inverse([A])[0]
+    assert np.allclose(X, np.linalg.inv(A), atol=1e-13)
+
+
+def _match_eigs(w, ref):
+    ref = list(ref)
+    errs = []
+    for v in w:
+        i = int(np.argmin([abs(v - r) for r in ref]))
+        errs.append(abs(v - ref[i]))
+        ref.pop(i)
+    return max(errs)
+
+
+@pytest.mark.parametrize("n", [2, 3, 8, 40, 98, 242])
+def test_eig_residual_and_values(ctx, n):
+    rng = np.random.default_rng(100 + n)
+    A = [rnd(rng, n, n) for _ in range(3)]
+    w, V = ctx.eig(A)
+    for a, wi, vi in zip(A, w, V):
+        nrm = np.abs(a).max()
+        res = np.abs(a @ vi - vi * wi[None, :]).max() / nrm
+        assert res < 1e-12 * n, res
+        assert np.allclose(np.linalg.norm(vi, axis=0), 1.0, atol=1e-12)
+        assert _match_eigs(wi, np.linalg.eigvals(a)) < 1e-10 * nrm
+
+
+def test_eig_degenerate_diagonal(ctx):
+    # a uniform layer gives a diagonal Omega^2 with every eigenvalue twice
+    d = np.repeat(np.arange(1, 6) - 3.5 + 0.1j, 2)
+    w, V = ctx.eig([np.diag(d)])
+    assert _match_eigs(w[0], d) < 1e-14
+    assert np.linalg.cond(V[0]) < 1e3
+
+
+@pytest.mark.parametrize("grid,hx,hy", [((8, 8), 1, 1), ((8, 6), 2, 1), ((16, 12), 2, 3),
+                                        ((64, 64), 5, 5)])
+def test_convmat_vs_oracle(ctx, grid, hx, hy):
+    rng = np.random.default_rng(7)
+    Ny, Nx = grid
+    g = rnd(rng, 2, Ny, Nx)
+    for layer in range(2):
+        c = ctx.convmat(g, layer, hx, hy)
+        ref = O.convmat(g[layer], hx, hy)
+        assert np.abs(c - ref).max() < 1e-14 * np.abs(g).max()

@@ -1,0 +1,100 @@
+"""End-to-end parity of the HIP solve path with the CPU FP64 oracle.
+
+Tolerances (from the FP64 contract): per-order and total R/T within 1e-9
+absolute of the oracle; R + T = 1 within 1e-10 for lossless stacks."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import rcwa_amd as G
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def _cmp(gpu, ref):
+    R, T, tot = gpu
+    Ro, To, toto = ref
+    return (np.abs(R - Ro).max(), np.abs(T - To).max(), np.abs(tot - toto).max())
+
+
+def test_layer_smatrix_vs_oracle(ctx):
+    rng = np.random.default_rng(3)
+    p = W.random_small(rng, hx=2, hy=2, L=2, grid=16)
+    k, Kx, Ky, Kzr, Kzt = O.wave_vectors(p.Lx, p.Ly, p.lam0, p.theta, p.phi, p.hx, p.hy,
+                                         p.er_ref, p.ur_ref, p.er_trn, p.ur_trn)
+    ercs = [O.convmat(p.er[l], p.hx, p.hy) for l in range(2)]
+    urcs = [O.convmat(p.ur[l], p.hx, p.hy) for l in range(2)]
+    k0 = 2 * np.pi / p.lam0
+    S11, S12 = ctx.layer_smatrix(ercs, urcs, Kx, Ky, k0 * p.thick)
+    for l in range(2):
+        r11, r12 = O.smatrix_layer(ercs[l], urcs[l], Kx, Ky, k0, p.thick[l])
+        assert np.abs(S11[l] - r11).max() < 1e-10
+        assert np.abs(S12[l] - r12).max() < 1e-10
+
+
+def test_star_product_vs_oracle(ctx):
+    rng = np.random.default_rng(5)
+    n = 18
+    A = tuple(0.3 * (rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n)))
+              for _ in range(4))
+    B = tuple(0.3 * (rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n)))
+              for _ in range(4))
+    S = ctx.star(A, B)
+    for b in range(2):
+        ref = O.redheffer([x[b] for x in A], [x[b] for x in B])
+        for k in range(4):
+            assert np.abs(S[k][b] - ref[k]).max() < 1e-12
+
+
+@pytest.mark.parametrize("seed,hx,hy,L,lossless", [(0, 1, 1, 1, True), (1, 2, 1, 3, False),
+                                                   (2, 2, 2, 2, True), (3, 3, 2, 4, False),
+                                                   (4, 0, 3, 2, False)])
+def test_solve_small_vs_oracle(ctx, seed, hx, hy, L, lossless):
+    rng = np.random.default_rng(seed)
+    p = W.random_small(rng, hx=hx, hy=hy, L=L, grid=16, lossless=lossless)
+    gpu = ctx.solve([p])[0]
+    ref = O.solve(p)
+    dR, dT, dtot = _cmp(gpu, ref)
+    assert max(dR, dT, dtot) < TOL, (dR, dT, dtot)
+    if lossless:
+        assert np.abs(gpu[2].sum(axis=1) - 1.0).max() < 1e-10
+
+
+def test_solve_zero_layers(ctx):
+    rng = np.random.default_rng(9)
+    p = W.random_small(rng, hx=1, hy=1, L=1, grid=8)
+    p.er = p.er[:0]
+    p.ur = p.ur[:0]
+    p.thick = p.thick[:0]
+    gpu = ctx.solve([p])[0]
+    ref = O.solve(p)
+    assert max(_cmp(gpu, ref)) < TOL
+
+
+def test_solve_batch_mixed_shapes(ctx):
+    rng = np.random.default_rng(11)
+    probs = [W.random_small(rng, hx=1, hy=1, L=2, grid=8),
+             W.random_small(rng, hx=2, hy=1, L=1, grid=8),
+             W.random_small(rng, hx=1, hy=1, L=2, grid=8)]
+    out = ctx.solve(probs)
+    for p, g in zip(probs, out):
+        assert max(_cmp(g, O.solve(p))) < TOL
+
+
+def test_c1_vs_oracle(ctx):
+    p = W.c1_single()
+    gpu = ctx.solve([p])[0]
+    ref = O.solve(p)
+    dR, dT, dtot = _cmp(gpu, ref)
+    assert max(dR, dT, dtot) < TOL, (dR, dT, dtot)
+
+
+def test_c1_lossless_energy(ctx):
+    p = W.c1_single()
+    p.er = p.er.real.astype(np.complex128)
+    gpu = ctx.solve([p])[0]
+    assert abs(gpu[2][0].sum() - 1.0) < 1e-10
+    ref = O.solve(p)
+    assert max(_cmp(gpu, ref)) < TOL
