@@ -52,7 +52,7 @@ size_t convmat_rowbuf_elems(int Nx, int Ny, int hx, int batch);
 // Toeplitz gather C[b](m, n) = F[b](dq, dp) (rcwa.cpp:56-78), zero outside
 // the grid's frequency range.
 int convmat_gather(hipStream_t s, const zc* F, int Nx, int Ny, int hx, int hy, ZMat C,
-                   int batch);
+                   int batch, const int* map = nullptr);  // dest slot map[b] (device) if given
 // P and Q of rcwa.cpp:264-285 from the inverses of [[eps]] and [[mu]].
 // Wave vectors Kx, Ky are per batch entry (stride N).
 int build_pq(hipStream_t s, int N, ZCMat erc, ZCMat urc, ZCMat erci, ZCMat urci, const zc* Kx,

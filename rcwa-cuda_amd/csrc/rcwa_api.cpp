@@ -226,6 +226,8 @@ rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], z
 // ------------------------------------------------------------------ solve pipeline
 namespace {
 
+// One S-matrix to compute: a device at one incidence, with the sources
+// (polarisations) that share it.
 struct Job {
   const rcwa_device* dev;
   const rcwa_params* par;
@@ -235,6 +237,7 @@ struct Job {
   const rcwa_source* src;  // problem's source array
   rcwa_results* out;       // problem's output array
   rcwa_host::WaveVectors wv;
+  std::vector<cplx> cols;  // S11, S21 columns at the source entries
 };
 
 rcwa_status check_device(const rcwa_device* d, const rcwa_params* p) {
@@ -251,61 +254,77 @@ rcwa_status check_device(const rcwa_device* d, const rcwa_params* p) {
 
 inline cplx C(rcwa_c128 z) { return cplx(z.re, z.im); }
 
-struct ChunkPlan {
-  int G, Bl;
-};
+}  // namespace
 
-size_t chunk_bytes(int N, int L, int Nx, int Ny, int hx, int hy, bool host_grids, int G, int Bl) {
-  const size_t n = 2 * (size_t)N, n2 = n * n, GL = (size_t)G * std::max(L, 1);
+// HBM bytes of one chunk: G S-matrices with TL layers in total, layer
+// sub-batch Bl.
+size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool host_grids,
+                   int G, int Bl) {
+  const size_t n = 2 * (size_t)N, n2 = n * n, T1 = std::max(TL, std::max(G, 1));
+  const size_t nf = (size_t)(4 * hx + 1) * (4 * hy + 1);
   size_t z = 0;
-  if (host_grids) z += 2 * GL * Nx * Ny;
-  z += GL * Ny * (4 * hx + 1);                  // dft rows
-  z += GL * (4 * hx + 1) * (4 * hy + 1);        // F
-  z += 2 * GL * N * N;                          // erc, urc
-  z += GL * (2 * N + 4 * N) + 4 * GL * n;       // Kx, Ky, v0, lam2/lam/ilam/X
-  z += 2 * GL * n2;                             // layer S11, S12
-  z += 13 * (size_t)G * n2;                     // fold slots
-  z += (size_t)G * 32 * N + (size_t)G * 4 * n;  // end diagonals, gathered columns
-  size_t bytes = z * sizeof(zc) + GL * sizeof(double) + 4 * GL * sizeof(int);
-  bytes += layer_work_bytes(N, Bl) + gj_workspace_bytes((int)n, std::max(G, Bl)) +
+  if (host_grids) z += 2 * (size_t)Lmax * Nx * Ny;  // one job's grids at a time
+  z += (size_t)Lmax * (Ny * (4 * hx + 1) + nf);    // DFT rows and coefficients
+  z += 2 * T1 * N * N;                              // erc, urc
+  z += T1 * (6 * (size_t)N) + 4 * T1 * n;           // Kx, Ky, v0, lam2/lam/ilam/X
+  z += 2 * T1 * n2;                                 // layer S11, S12
+  z += 13 * (size_t)G * n2;                         // fold slots
+  z += (size_t)G * 32 * N + (size_t)G * 4 * n;      // end diagonals, gathered columns
+  size_t bytes = z * sizeof(zc) + T1 * (sizeof(double) + 3 * sizeof(int)) + 64;
+  bytes += layer_work_bytes(N, Bl) + gj_workspace_bytes((int)n, std::max(G, 1)) +
            eig_workspace_bytes((int)n, Bl) + 64 * 256;
   return bytes;
 }
 
-}  // namespace
-
-rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_out) {
+// Computes the S-matrices and efficiencies of G jobs that share (hx, hy,
+// Nx, Ny).  Jobs may have different layer counts: they are sorted by layer
+// count (descending) and their layers are laid out layer-major, so that at
+// fold step l the jobs still folding are a prefix of the job list and their
+// layer-l S-matrices are contiguous.
+static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
+  std::stable_sort(jobs.begin(), jobs.end(),
+                   [](const Job* a, const Job* b) { return a->dev->num_layers > b->dev->num_layers; });
   const int G = (int)jobs.size();
   const rcwa_device* d0 = jobs[0]->dev;
   const rcwa_params* p0 = jobs[0]->par;
   const int hx = p0->Nx_harmonics, hy = p0->Ny_harmonics;
   const int P = 2 * hx + 1, Q = 2 * hy + 1, N = P * Q, n = 2 * N;
-  const int L = d0->num_layers, Nx = d0->Nx, Ny = d0->Ny;
+  const int Nx = d0->Nx, Ny = d0->Ny;
   const bool host_grids = !d0->grids_on_device;
-  const int GL = G * L;
+  const int Lmax = d0->num_layers;
+  std::vector<int> act(Lmax + 1, 0), off(Lmax + 1, 0);  // active jobs and offsets per layer
+  for (int l = 0; l < Lmax; ++l) {
+    for (int g = 0; g < G; ++g) act[l] += jobs[g]->dev->num_layers > l;
+    off[l + 1] = off[l] + act[l];
+  }
+  const int TL = off[Lmax];
   const long long n2 = (long long)n * n, N2 = (long long)N * N;
+  const int nfx = 4 * hx + 1, nfy = 4 * hy + 1;
   hipStream_t s = c.stream;
 
-  // pick the layer sub-batch that fits the budget
-  int Bl = std::max(GL, 1);
-  while (Bl > 1 && chunk_bytes(N, L, Nx, Ny, hx, hy, host_grids, G, Bl) > c.budget) Bl = (Bl + 1) / 2;
-  rcwa_status st = c.reserve(chunk_bytes(N, L, Nx, Ny, hx, hy, host_grids, G, Bl));
+  int Bl = std::max(TL, 1);
+  while (Bl > 1 && chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) > c.budget)
+    Bl = (Bl + 1) / 2;
+  rcwa_status st = c.reserve(chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl));
   if (st) return st;
 
   // ---- host precompute (O(N) per job)
-  std::vector<cplx> hKx((size_t)GL * N), hKy((size_t)GL * N), hv0((size_t)GL * 4 * N);
-  std::vector<double> hk0t(std::max(GL, 1));
+  const size_t T1 = std::max(TL, std::max(G, 1));
+  std::vector<cplx> hKx((size_t)T1 * N), hKy((size_t)T1 * N), hv0((size_t)T1 * 4 * N);
+  std::vector<double> hk0t(T1, 0.0);
+  std::vector<int> hmap(T1, 0);
   std::vector<cplx> hends((size_t)G * 32 * N);
   for (int g = 0; g < G; ++g) {
     Job& j = *jobs[g];
     j.wv = rcwa_host::wave_vectors(j.dev->Lx, j.dev->Ly, j.inc, hx, hy, j.media);
     std::vector<cplx> v0 = rcwa_host::gap_v0_diagonals(j.wv.Kx, j.wv.Ky);
     const double k0 = 2.0 * M_PI / j.inc.lambda0;
-    for (int l = 0; l < L; ++l) {
-      std::copy(j.wv.Kx.begin(), j.wv.Kx.end(), hKx.begin() + ((size_t)g * L + l) * N);
-      std::copy(j.wv.Ky.begin(), j.wv.Ky.end(), hKy.begin() + ((size_t)g * L + l) * N);
-      std::copy(v0.begin(), v0.end(), hv0.begin() + ((size_t)g * L + l) * 4 * N);
-      hk0t[(size_t)g * L + l] = k0 * j.dev->t[l];
+    for (int l = 0; l < j.dev->num_layers; ++l) {
+      const size_t li = (size_t)off[l] + g;
+      std::copy(j.wv.Kx.begin(), j.wv.Kx.end(), hKx.begin() + li * N);
+      std::copy(j.wv.Ky.begin(), j.wv.Ky.end(), hKy.begin() + li * N);
+      std::copy(v0.begin(), v0.end(), hv0.begin() + li * 4 * N);
+      hk0t[li] = k0 * j.dev->t[l];
     }
     std::vector<cplx> er = rcwa_host::end_smatrix_diagonals(0, j.wv.Kx, j.wv.Ky, j.wv.Kz_ref,
                                                            j.media.er_ref, j.media.ur_ref);
@@ -316,45 +335,53 @@ rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_o
   }
 
   // ---- device buffers
-  const size_t G1 = std::max(GL, std::max(G, 1));
   zc* dgr_e = nullptr;
   zc* dgr_u = nullptr;
-  if (host_grids && L > 0) {
-    dgr_e = (zc*)c.take((size_t)GL * Nx * Ny * sizeof(zc));
-    dgr_u = (zc*)c.take((size_t)GL * Nx * Ny * sizeof(zc));
+  if (host_grids && Lmax > 0) {
+    dgr_e = (zc*)c.take((size_t)Lmax * Nx * Ny * sizeof(zc));
+    dgr_u = (zc*)c.take((size_t)Lmax * Nx * Ny * sizeof(zc));
   }
-  zc* rowbuf = (zc*)c.take(G1 * Ny * (4 * hx + 1) * sizeof(zc));
-  zc* F = (zc*)c.take(G1 * (4 * hx + 1) * (4 * hy + 1) * sizeof(zc));
-  zc* erc = (zc*)c.take(G1 * N2 * sizeof(zc));
-  zc* urc = (zc*)c.take(G1 * N2 * sizeof(zc));
-  zc* dKx = (zc*)c.take(G1 * N * sizeof(zc));
-  zc* dKy = (zc*)c.take(G1 * N * sizeof(zc));
-  zc* dv0 = (zc*)c.take(G1 * 4 * N * sizeof(zc));
-  double* dk0t = (double*)c.take(G1 * sizeof(double));
-  zc* lam2 = (zc*)c.take(G1 * n * sizeof(zc));
-  zc* lam = (zc*)c.take(G1 * n * sizeof(zc));
-  zc* ilam = (zc*)c.take(G1 * n * sizeof(zc));
-  zc* Xd = (zc*)c.take(G1 * n * sizeof(zc));
-  zc* S11l = (zc*)c.take(G1 * n2 * sizeof(zc));
-  zc* S12l = (zc*)c.take(G1 * n2 * sizeof(zc));
+  zc* rowbuf = (zc*)c.take((size_t)std::max(Lmax, 1) * Ny * nfx * sizeof(zc));
+  zc* F = (zc*)c.take((size_t)std::max(Lmax, 1) * nfx * nfy * sizeof(zc));
+  zc* erc = (zc*)c.take(T1 * N2 * sizeof(zc));
+  zc* urc = (zc*)c.take(T1 * N2 * sizeof(zc));
+  zc* dKx = (zc*)c.take(T1 * N * sizeof(zc));
+  zc* dKy = (zc*)c.take(T1 * N * sizeof(zc));
+  zc* dv0 = (zc*)c.take(T1 * 4 * N * sizeof(zc));
+  double* dk0t = (double*)c.take(T1 * sizeof(double));
+  int* dmap = (int*)c.take((size_t)(Lmax + 1) * G * sizeof(int) + 64);
+  zc* lam2 = (zc*)c.take(T1 * n * sizeof(zc));
+  zc* lam = (zc*)c.take(T1 * n * sizeof(zc));
+  zc* ilam = (zc*)c.take(T1 * n * sizeof(zc));
+  zc* Xd = (zc*)c.take(T1 * n * sizeof(zc));
+  zc* S11l = (zc*)c.take(T1 * n2 * sizeof(zc));
+  zc* S12l = (zc*)c.take(T1 * n2 * sizeof(zc));
   zc* slot[13];
   for (int i = 0; i < 13; ++i) slot[i] = (zc*)c.take((size_t)G * n2 * sizeof(zc));
   zc* dends = (zc*)c.take((size_t)G * 32 * N * sizeof(zc));
   zc* dcols = (zc*)c.take((size_t)G * 4 * n * sizeof(zc));
-  int* dinfo = (int*)c.take(4 * G1 * sizeof(int));
+  int* dinfo = (int*)c.take(2 * T1 * sizeof(int));
   int* dcolidx = (int*)c.take(4 * sizeof(int));
   void* gjf = c.take(gj_workspace_bytes(n, std::max(G, 1)));
   LayerWork w;
   w.carve(c, N, Bl);
+  if (c.arena_off > c.arena_cap)
+    return rcwa_set_error(RCWA_E_NOMEM, "internal: arena plan exceeded its reservation");
 
-  CKH(hipMemsetAsync(dinfo, 0, 4 * G1 * sizeof(int), s));
+  // layer-major destination of job g's layer l
+  std::vector<int> jmap((size_t)std::max(Lmax, 1) * G, 0);
+  for (int g = 0; g < G; ++g)
+    for (int l = 0; l < jobs[g]->dev->num_layers; ++l) jmap[(size_t)g * Lmax + l] = off[l] + g;
+
+  CKH(hipMemsetAsync(dinfo, 0, 2 * T1 * sizeof(int), s));
   int* info_eig = dinfo;
-  int* info_sing = dinfo + G1;
-  if (L > 0) {
-    CKH(hipMemcpyAsync(dKx, hKx.data(), (size_t)GL * N * sizeof(zc), hipMemcpyHostToDevice, s));
-    CKH(hipMemcpyAsync(dKy, hKy.data(), (size_t)GL * N * sizeof(zc), hipMemcpyHostToDevice, s));
-    CKH(hipMemcpyAsync(dv0, hv0.data(), (size_t)GL * 4 * N * sizeof(zc), hipMemcpyHostToDevice, s));
-    CKH(hipMemcpyAsync(dk0t, hk0t.data(), (size_t)GL * sizeof(double), hipMemcpyHostToDevice, s));
+  int* info_sing = dinfo + T1;
+  if (TL > 0) {
+    CKH(hipMemcpyAsync(dKx, hKx.data(), (size_t)TL * N * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dKy, hKy.data(), (size_t)TL * N * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dv0, hv0.data(), (size_t)TL * 4 * N * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dk0t, hk0t.data(), (size_t)TL * sizeof(double), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dmap, jmap.data(), jmap.size() * sizeof(int), hipMemcpyHostToDevice, s));
   }
   CKH(hipMemcpyAsync(dends, hends.data(), (size_t)G * 32 * N * sizeof(zc), hipMemcpyHostToDevice, s));
   const int hcol[4] = {N / 2, N / 2 + N, 0, 0};
@@ -363,37 +390,32 @@ rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_o
   c.ev_on = true;
   CKH(hipEventRecord(c.ev[0], s));
   // ---- ConvMat for every layer (main.cpp:126-138)
-  if (L > 0) {
-    const size_t G2 = (size_t)Nx * Ny;
-    for (int g = 0; g < G; ++g) {
-      const rcwa_device* d = jobs[g]->dev;
-      const zc *ge, *gu;
-      if (host_grids) {
-        CKH(hipMemcpyAsync(dgr_e + (size_t)g * L * G2, d->er, (size_t)L * G2 * sizeof(zc),
-                           hipMemcpyHostToDevice, s));
-        CKH(hipMemcpyAsync(dgr_u + (size_t)g * L * G2, d->ur, (size_t)L * G2 * sizeof(zc),
-                           hipMemcpyHostToDevice, s));
-        ge = dgr_e + (size_t)g * L * G2;
-        gu = dgr_u + (size_t)g * L * G2;
-      } else {
-        ge = (const zc*)d->er;
-        gu = (const zc*)d->ur;
-      }
-      CK(convmat_dft(s, ge, Nx, Ny, hx, hy, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), L,
-                     rowbuf + (size_t)g * L * Ny * (4 * hx + 1)));
-      CK(convmat_gather(s, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), Nx, Ny, hx, hy,
-                        ZMat{erc + (size_t)g * L * N2, N, N2}, L));
-      CK(convmat_dft(s, gu, Nx, Ny, hx, hy, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), L,
-                     rowbuf + (size_t)g * L * Ny * (4 * hx + 1)));
-      CK(convmat_gather(s, F + (size_t)g * L * (4 * hx + 1) * (4 * hy + 1), Nx, Ny, hx, hy,
-                        ZMat{urc + (size_t)g * L * N2, N, N2}, L));
+  const size_t G2 = (size_t)Nx * Ny;
+  for (int g = 0; g < G; ++g) {
+    const rcwa_device* d = jobs[g]->dev;
+    const int L = d->num_layers;
+    if (L == 0) continue;
+    const zc *ge, *gu;
+    if (host_grids) {
+      CKH(hipMemcpyAsync(dgr_e, d->er, (size_t)L * G2 * sizeof(zc), hipMemcpyHostToDevice, s));
+      CKH(hipMemcpyAsync(dgr_u, d->ur, (size_t)L * G2 * sizeof(zc), hipMemcpyHostToDevice, s));
+      ge = dgr_e;
+      gu = dgr_u;
+    } else {
+      ge = (const zc*)d->er;
+      gu = (const zc*)d->ur;
     }
+    const int* map = dmap + (size_t)g * Lmax;
+    CK(convmat_dft(s, ge, Nx, Ny, hx, hy, F, L, rowbuf));
+    CK(convmat_gather(s, F, Nx, Ny, hx, hy, ZMat{erc, N, N2}, L, map));
+    CK(convmat_dft(s, gu, Nx, Ny, hx, hy, F, L, rowbuf));
+    CK(convmat_gather(s, F, Nx, Ny, hx, hy, ZMat{urc, N, N2}, L, map));
   }
   CKH(hipEventRecord(c.ev[1], s));
   // ---- SMatrixLayer for every layer (main.cpp:151)
   float ms_eig = 0;
-  for (int l0 = 0; l0 < GL; l0 += Bl) {
-    const int b = std::min(Bl, GL - l0);
+  for (int l0 = 0; l0 < TL; l0 += Bl) {
+    const int b = std::min(Bl, TL - l0);
     st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
                      dKx + (size_t)l0 * N, dKy + (size_t)l0 * N, dv0 + (size_t)l0 * 4 * N,
                      dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
@@ -401,34 +423,36 @@ rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_o
                      ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
                      info_eig + l0, info_sing + l0);
     if (st) return st;
-    CKH(hipEventSynchronize(c.ev[3]));
-    float t = 0;
-    CKH(hipEventElapsedTime(&t, c.ev[2], c.ev[3]));
-    ms_eig += t;
+    if (c.time_eig) {
+      CKH(hipEventSynchronize(c.ev[3]));
+      float t = 0;
+      CKH(hipEventElapsedTime(&t, c.ev[2], c.ev[3]));
+      ms_eig += t;
+    }
   }
   CKH(hipEventRecord(c.ev[4], s));
   // ---- fold (main.cpp:143-164), ends (main.cpp:166-170)
   zc* S[4] = {slot[0], slot[1], slot[2], slot[3]};
   zc* E[4] = {slot[4], slot[5], slot[6], slot[7]};
   zc* T[5] = {slot[8], slot[9], slot[10], slot[11], slot[12]};
-  if (L == 0) {
-    // SMatrixInit (rcwa.cpp:237-250)
-    CK(zset_identity(s, n, ZMat{S[0], n, n2}, G, zmake(0, 0)));
-    CK(zset_identity(s, n, ZMat{S[1], n, n2}, G, zmake(1, 0)));
-    CK(zset_identity(s, n, ZMat{S[2], n, n2}, G, zmake(1, 0)));
-    CK(zset_identity(s, n, ZMat{S[3], n, n2}, G, zmake(0, 0)));
-  } else {
-    // SMatrixInit * S_0 = S_0 exactly in IEEE arithmetic (every product is
-    // with I or 0), so the fold starts from S_0.
-    const long long lstride = (long long)L * n2;
-    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, lstride}, ZMat{S[0], n, n2}, G));
-    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, lstride}, ZMat{S[1], n, n2}, G));
-    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, lstride}, ZMat{S[2], n, n2}, G));
-    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, lstride}, ZMat{S[3], n, n2}, G));
-    for (int l = 1; l < L; ++l) {
-      const ZCMat B[4] = {{S11l + (size_t)l * n2, n, lstride}, {S12l + (size_t)l * n2, n, lstride},
-                          {S12l + (size_t)l * n2, n, lstride}, {S11l + (size_t)l * n2, n, lstride}};
-      st = star_product(c, n, G, S, B, T, gjf, info_sing);
+  // SMatrixInit (rcwa.cpp:237-250) for every job; then, for the jobs with
+  // layers, SMatrixInit * S_0 = S_0 exactly in IEEE arithmetic (every
+  // product is with I or 0), so their fold starts from S_0.
+  CK(zset_identity(s, n, ZMat{S[0], n, n2}, G, zmake(0, 0)));
+  CK(zset_identity(s, n, ZMat{S[1], n, n2}, G, zmake(1, 0)));
+  CK(zset_identity(s, n, ZMat{S[2], n, n2}, G, zmake(1, 0)));
+  CK(zset_identity(s, n, ZMat{S[3], n, n2}, G, zmake(0, 0)));
+  if (Lmax > 0) {
+    const int a0 = act[0];
+    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, n2}, ZMat{S[0], n, n2}, a0));
+    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, n2}, ZMat{S[1], n, n2}, a0));
+    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, n2}, ZMat{S[2], n, n2}, a0));
+    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, n2}, ZMat{S[3], n, n2}, a0));
+    for (int l = 1; l < Lmax; ++l) {
+      const zc* b11 = S11l + (size_t)off[l] * n2;
+      const zc* b12 = S12l + (size_t)off[l] * n2;
+      const ZCMat B[4] = {{b11, n, n2}, {b12, n, n2}, {b12, n, n2}, {b11, n, n2}};
+      st = star_product(c, n, act[l], S, B, T, gjf, info_sing);
       if (st) return st;
     }
   }
@@ -454,8 +478,8 @@ rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_o
   CK(gather_cols(s, n, ZCMat{E[2], n, n2}, dcolidx, 2, dcols + (size_t)G * 2 * n, G));
   std::vector<cplx> hcols((size_t)G * 4 * n);
   CKH(hipMemcpyAsync(hcols.data(), dcols, hcols.size() * sizeof(zc), hipMemcpyDeviceToHost, s));
-  std::vector<int> hinfo(2 * G1);
-  CKH(hipMemcpyAsync(hinfo.data(), dinfo, 2 * G1 * sizeof(int), hipMemcpyDeviceToHost, s));
+  std::vector<int> hinfo(2 * T1);
+  CKH(hipMemcpyAsync(hinfo.data(), dinfo, 2 * T1 * sizeof(int), hipMemcpyDeviceToHost, s));
   CKH(hipEventRecord(c.ev[6], s));
   CKH(hipStreamSynchronize(s));
   c.ev_on = false;
@@ -469,25 +493,24 @@ rcwa_status run_chunk(Ctx& c, std::vector<Job*>& jobs, std::vector<cplx>* cols_o
   c.stats.ms_eig += ms_eig;
   c.stats.ms_fold += t45;
   c.stats.ms_out += t56;
+  c.stats.ms_total += t01 + t14 + t45 + t56;
   c.stats.n_smatrices += G;
-  c.stats.n_layers += GL;
+  c.stats.n_layers += TL;
   const double N3 = (double)N * N * N;
-  c.stats.flops_algorithmic += (2384.0 * GL + 1621.0 * G) * N3;
+  c.stats.flops_algorithmic += (2384.0 * TL + 1621.0 * G) * N3;
 
-  for (int g = 0; g < GL; ++g)
-    if (hinfo[g]) return rcwa_set_error(RCWA_E_EIG_NOCONV, "eigensolver QR did not converge");
-  for (size_t g = 0; g < G1; ++g)
-    if (hinfo[G1 + g]) return rcwa_set_error(RCWA_E_SINGULAR, "exactly singular matrix inverted");
+  for (int i = 0; i < TL; ++i)
+    if (hinfo[i]) return rcwa_set_error(RCWA_E_EIG_NOCONV, "eigensolver QR did not converge");
+  for (size_t i = 0; i < T1; ++i)
+    if (hinfo[T1 + i]) return rcwa_set_error(RCWA_E_SINGULAR, "exactly singular matrix inverted");
 
   // ---- efficiencies per source (host, O(N))
   for (int g = 0; g < G; ++g) {
     Job& j = *jobs[g];
     const cplx* s11c = hcols.data() + (size_t)g * 2 * n;
     const cplx* s21c = hcols.data() + (size_t)G * 2 * n + (size_t)g * 2 * n;
-    if (cols_out) {
-      cols_out->insert(cols_out->end(), s11c, s11c + 2 * n);
-      cols_out->insert(cols_out->end(), s21c, s21c + 2 * n);
-    }
+    j.cols.assign(s11c, s11c + 2 * n);
+    j.cols.insert(j.cols.end(), s21c, s21c + 2 * n);
     for (int si : j.srcs) {
       const rcwa_source& src = j.src[si];
       rcwa_host::Efficiencies e =
@@ -541,28 +564,41 @@ rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_prec
       }
     }
   }
-  // group by shape
-  std::map<std::tuple<int, int, int, int, int, int>, std::vector<Job*>> groups;
+  // bucket by shape; layer counts may differ within a bucket
+  std::map<std::tuple<int, int, int, int, int>, std::vector<Job*>> groups;
   for (auto& j : jobs)
-    groups[{j.par->Nx_harmonics, j.par->Ny_harmonics, j.dev->num_layers, j.dev->Nx, j.dev->Ny,
+    groups[{j.par->Nx_harmonics, j.par->Ny_harmonics, j.dev->Nx, j.dev->Ny,
             j.dev->grids_on_device}]
         .push_back(&j);
   for (auto& kv : groups) {
-    std::vector<Job*>& v = kv.second;
+    std::vector<Job*> v = kv.second;
+    std::stable_sort(v.begin(), v.end(), [](const Job* a, const Job* b) {
+      return a->dev->num_layers > b->dev->num_layers;
+    });
     const Job& j0 = *v[0];
     const int hx = j0.par->Nx_harmonics, hy = j0.par->Ny_harmonics;
     const int N = (2 * hx + 1) * (2 * hy + 1);
-    int G = (int)v.size();
-    if (c.max_chunk > 0) G = std::min(G, c.max_chunk);
-    while (G > 1 && chunk_bytes(N, j0.dev->num_layers, j0.dev->Nx, j0.dev->Ny, hx, hy,
-                                !j0.dev->grids_on_device, G, 1) > c.budget)
-      G = (G + 1) / 2;
-    for (size_t i = 0; i < v.size(); i += G) {
-      std::vector<Job*> part(v.begin() + i, v.begin() + std::min(v.size(), i + (size_t)G));
-      rcwa_status st = run_chunk(c, part, &c.last_cols);
+    size_t i = 0;
+    while (i < v.size()) {
+      // largest prefix (at most max_chunk jobs) whose chunk fits the budget
+      // with a one-layer sub-batch
+      size_t G = v.size() - i;
+      if (c.max_chunk > 0) G = std::min(G, (size_t)c.max_chunk);
+      for (;;) {
+        int TL = 0;
+        for (size_t k = i; k < i + G; ++k) TL += v[k]->dev->num_layers;
+        const size_t need = chunk_bytes(N, TL, v[i]->dev->num_layers, j0.dev->Nx, j0.dev->Ny, hx,
+                                        hy, !j0.dev->grids_on_device, (int)G, 1);
+        if (need <= c.budget || G == 1) break;
+        G = (G + 1) / 2;
+      }
+      std::vector<Job*> part(v.begin() + i, v.begin() + i + G);
+      rcwa_status st = run_chunk(c, part);
       if (st) return st;
+      i += G;
     }
   }
+  for (auto& j : jobs) c.last_cols.insert(c.last_cols.end(), j.cols.begin(), j.cols.end());
   return RCWA_OK;
 }
 

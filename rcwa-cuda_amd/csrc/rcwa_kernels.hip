@@ -67,12 +67,13 @@ __global__ __launch_bounds__(DFT_THREADS) void dft_cols_kernel(const zc* R, int 
   }
 }
 
-__global__ void gather_kernel(const zc* F, int Nx, int Ny, int hx, int hy, ZMat C) {
+__global__ void gather_kernel(const zc* F, int Nx, int Ny, int hx, int hy, ZMat C,
+                              const int* map) {
   const int P = 2 * hx + 1, Q = 2 * hy + 1, PQ = P * Q;
   const int nfx = 4 * hx + 1, nfy = 4 * hy + 1;
   const int b = blockIdx.y;
   const zc* Fb = F + (long long)b * nfx * nfy;
-  zc* Cb = C.p + (long long)b * C.stride;
+  zc* Cb = C.p + (long long)(map ? map[b] : b) * C.stride;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)PQ * PQ;
        idx += (long long)gridDim.x * blockDim.x) {
     const int m = (int)(idx % PQ), n = (int)(idx / PQ);
@@ -195,9 +196,9 @@ int convmat_dft(hipStream_t s, const zc* grids, int Nx, int Ny, int hx, int hy, 
 }
 
 int convmat_gather(hipStream_t s, const zc* F, int Nx, int Ny, int hx, int hy, ZMat C,
-                   int batch) {
+                   int batch, const int* map) {
   const long long PQ = (long long)(2 * hx + 1) * (2 * hy + 1);
-  hipLaunchKernelGGL(gather_kernel, grid2(PQ * PQ, batch), dim3(256), 0, s, F, Nx, Ny, hx, hy, C);
+  hipLaunchKernelGGL(gather_kernel, grid2(PQ * PQ, batch), dim3(256), 0, s, F, Nx, Ny, hx, hy, C, map);
   return (int)hipGetLastError();
 }
 

@@ -28,6 +28,7 @@ struct Ctx {
   std::vector<std::complex<double>> last_cols;
   hipEvent_t ev[8] = {};
   bool ev_on = false;
+  bool time_eig = false;  // synchronise after each layer sub-batch to time the eigensolver
 
   rcwa_status init(int dev, size_t budget);
   void release();
@@ -47,6 +48,8 @@ struct LayerWork {
 };
 
 size_t layer_work_bytes(int N, int Bl);
+size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool host_grids,
+                   int G, int Bl);
 
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,

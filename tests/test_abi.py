@@ -1,0 +1,102 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/rcwa_amd.h declares, and its host-side (O(N)) stages agree with the
+oracle: ComputeWaveVectors, GapMedium, SMatrixReflection/Transmission and
+the source/field/efficiency tail."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import np_rcwa as NR
+import oracle_py as O
+import rcwa_amd as G
+import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "rcwa_amd.h")).read()
+    return sorted(set(re.findall(r"\b(rcwa_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_function():
+    lib = G.lib()
+    decl = declared_functions()
+    assert len(decl) >= 25
+    missing = [f for f in decl if not hasattr(lib, f)]
+    assert not missing, missing
+    assert sorted(G.EXPORTS) == decl
+
+
+def test_no_gpu_context_fails_cleanly():
+    import ctypes
+    h = ctypes.c_void_p()
+    try:
+        import torch  # noqa: F401
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if has_gpu:
+        pytest.skip("a GPU is present")
+    rc = G.lib().rcwa_ctx_create(0, 0, ctypes.byref(h))
+    assert rc == 6 and b"no HIP device" in G.lib().rcwa_last_error()
+
+
+def test_host_wave_vectors_match_oracle():
+    args = (480.0, 350.0, 640.0, 0.5, 0.25, 3, 2, 1.2 - 0.05j, 1.0, 2.25, 1.0)
+    a = G.wave_vectors(*args)
+    b = O.wave_vectors(*args)
+    for x, y in zip(a, b):
+        assert np.abs(x - y).max() < 1e-15
+
+
+def test_host_gap_medium_matches_numpy():
+    k, Kx, Ky, *_ = O.wave_vectors(300.0, 280.0, 500.0, 0.3, 0.1, 2, 2, 1, 1, 2.25, 1)
+    W0, V0 = G.gap_medium(Kx, Ky)
+    W0n, V0n = NR.gap_medium(Kx, Ky)
+    assert np.array_equal(W0, W0n)
+    assert np.abs(V0 - V0n).max() < 1e-14 * np.abs(V0n).max()
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_host_end_smatrix_matches_dense_restatement(which):
+    args = (300.0, 280.0, 500.0, 0.3, 0.1, 2, 1, 1.0, 1.0, 2.1 - 0.2j, 1.0)
+    k, Kx, Ky, Kzr, Kzt = NR.wave_vectors(*args)
+    W0, V0 = NR.gap_medium(Kx, Ky)
+    if which == 0:
+        ref, _ = NR.smatrix_reflection(Kx, Ky, Kzr, 1.0, 1.0, W0, V0)
+        S = G.end_smatrix(0, 2, 1, Kx, Ky, Kzr, 1.0, 1.0, 2.1 - 0.2j, 1.0)
+    else:
+        ref, _ = NR.smatrix_transmission(Kx, Ky, Kzt, 2.1 - 0.2j, 1.0, W0, V0)
+        S = G.end_smatrix(1, 2, 1, Kx, Ky, Kzt, 1.0, 1.0, 2.1 - 0.2j, 1.0)
+    for a, b in zip(S, ref):
+        assert np.abs(a - b).max() < 1e-13
+
+
+def test_host_efficiencies_match_oracle_solve():
+    """The tail (source, fields, efficiencies) applied to the oracle's own
+    global S-matrix reproduces the oracle's R and T."""
+    import ctypes
+    rng = np.random.default_rng(12)
+    p = W.random_small(rng, hx=2, hy=1, L=2, grid=8)
+    R, T, tot, Sg = O.solve(p, return_smatrix=True)
+    N = p.N
+    c = [N // 2, N // 2 + N]
+    s11c = np.ascontiguousarray(np.concatenate([Sg[0][:, c[0]], Sg[0][:, c[1]]]))
+    s21c = np.ascontiguousarray(np.concatenate([Sg[2][:, c[0]], Sg[2][:, c[1]]]))
+    k, Kx, Ky, Kzr, Kzt = O.wave_vectors(p.Lx, p.Ly, p.lam0, p.theta, p.phi, p.hx, p.hy,
+                                         p.er_ref, p.ur_ref, p.er_trn, p.ur_trn)
+    par = G.Params(p.hx, p.hy, G._c(p.er_ref), G._c(p.ur_ref), G._c(p.er_trn), G._c(p.ur_trn))
+    for i, (a, b) in enumerate(p.pols):
+        src = G.Source(p.lam0, p.theta, p.phi, a, b)
+        Rh = np.zeros(N)
+        Th = np.zeros(N)
+        res = G.Results(Rh.ctypes.data, Th.ctypes.data, 0.0, 0.0)
+        G._check(G.lib().rcwa_efficiencies(ctypes.byref(src), ctypes.byref(par), G._ptr(k),
+                                           G._ptr(Kx), G._ptr(Ky), G._ptr(Kzr), G._ptr(Kzt),
+                                           G._ptr(s11c), G._ptr(s21c), ctypes.byref(res)))
+        assert np.abs(Rh.reshape(2 * p.hx + 1, -1).T - R[i]).max() < 1e-13
+        assert np.abs(Th.reshape(2 * p.hx + 1, -1).T - T[i]).max() < 1e-13
+        assert abs(res.R_tot - tot[i][0]) < 1e-13 and abs(res.T_tot - tot[i][1]) < 1e-13
