@@ -1,0 +1,226 @@
+"""Benchmark of the MI355X RCWA solve path on the 21x21-harmonic, 20-layer
+spectral sweep (1-D trapezoid line grating staircased into 20 slices over 5
+dispersive materials, 512x512 grids, lambda 250-800 nm in 200 points,
+theta = 65 deg, TE + TM sharing one S-matrix per wavelength).
+
+One step = one batched pass of the plain FP64 path over `--lams` wavelengths
+of the sweep (HBM-resident permittivity grids -> per-order R/T on the host).
+The JSON line reports:
+  value        solves/s (one solve = one wavelength with both polarisations)
+  latency_ms   p50/p99 of the per-step wall time (every solve of a step
+               completes with the step)
+  roofline     the QR-sweep kernel of the eigensolver (the dominant kernel),
+               its algorithmic FP64 flops per launch over its event-timed
+               average duration, against the FP64 matrix peak
+  cpu_baseline the CPU FP64 oracle on a bounded sample (one layer S-matrix
+               and one Redheffer product of the same workload, one thread
+               each), extrapolated to a full solve
+  split        (--split) the split-precision form on the same wavelengths:
+               solves/s and max |dR|, |dT| against this FP64 run
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
+
+METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def cpu_sample(which):
+    """Times one oracle stage at the sweep's size (runs in a child process)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    import workloads as W
+    p = W.c2_sweep(indices=[0])[0]
+    k, Kx, Ky, Kzr, Kzt = O.wave_vectors(p.Lx, p.Ly, p.lam0, p.theta, p.phi, p.hx, p.hy,
+                                         p.er_ref, p.ur_ref, p.er_trn, p.ur_trn)
+    erc = O.convmat(p.er[10], p.hx, p.hy)
+    urc = O.convmat(p.ur[10], p.hx, p.hy)
+    t0 = time.perf_counter()
+    S11, S12 = O.smatrix_layer(erc, urc, Kx, Ky, 2 * math.pi / p.lam0, p.thick[10])
+    t1 = time.perf_counter()
+    if which == "star":
+        S = (S11, S12, S12, S11)
+        t0 = time.perf_counter()
+        O.redheffer(S, S)
+        t1 = time.perf_counter()
+    print(json.dumps({"which": which, "seconds": t1 - t0}))
+
+
+class Hip:
+    """Minimal HIP runtime access for HBM-resident inputs."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so")
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        rc = self.lib.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError("hipMalloc failed: %d" % rc)
+        return p.value
+
+    def upload(self, dst, arr):
+        rc = self.lib.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(arr.ctypes.data),
+                                ctypes.c_size_t(arr.nbytes), 1)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+    def sync(self):
+        self.lib.hipDeviceSynchronize()
+
+    def free(self, p):
+        self.lib.hipFree(ctypes.c_void_p(p))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lams", type=int, default=8, help="wavelengths per step")
+    ap.add_argument("--split", action="store_true", help="also time the split-precision form")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", choices=["layer", "star"], help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_sample:
+        cpu_sample(args.cpu_sample)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the multi-GPU form of this path is design-only "
+              "(wavelengths/requests shard with no exchange); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import rcwa_amd as G
+    import workloads as W
+
+    nlam_needed = (args.warmup + args.steps) * args.lams
+    if nlam_needed > 200:
+        raise SystemExit("steps*lams exceeds the 200-point sweep")
+
+    # CPU baseline: two single-threaded oracle processes, concurrent with the GPU run
+    cpu_procs = []
+    if not args.no_cpu_baseline:
+        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1")
+        for which in ("layer", "star"):
+            cpu_procs.append(subprocess.Popen([sys.executable, os.path.abspath(__file__),
+                                               "--cpu-sample", which], stdout=subprocess.PIPE,
+                                              stderr=subprocess.PIPE, env=env, text=True))
+
+    # wavelengths spread over the sweep: warmup chunk first, then the timed chunks
+    idx = np.linspace(0, 199, nlam_needed).round().astype(int).tolist()
+    probs = W.c2_sweep(indices=idx)
+    hip = Hip()
+    ur_dev = hip.malloc(probs[0].ur.nbytes)
+    hip.upload(ur_dev, probs[0].ur)
+    er_devs = []
+    for p in probs:
+        d = hip.malloc(p.er.nbytes)
+        hip.upload(d, p.er)
+        er_devs.append(d)
+        p.er = p.er[:, :1, :1]  # host copy no longer needed; keep shape metadata small
+    hip.sync()
+
+    ctx = G.Context(0)
+
+    def run(chunk, precision=G.RCWA_FP64):
+        sel = probs[chunk * args.lams:(chunk + 1) * args.lams]
+        grids = [(er_devs[chunk * args.lams + i], ur_dev) for i in range(len(sel))]
+        return ctx.solve(sel, precision, device_grids=grids, grid_shape=(512, 512))
+
+    for w in range(args.warmup):
+        run(w)
+    hip.sync()
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    step_ms = []
+    results = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        results.append(run(args.warmup + k))
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+    hip.sync()
+    elapsed = time.perf_counter() - t_start
+    st = ctx.stats()
+    ctx.set_timing(False)
+    solves = args.steps * args.lams
+    value = solves / elapsed
+
+    N = 441
+    n = 2 * N
+    # QR sweeps with Schur vectors: (25 - 10/3 - 4/3) n^3 real flops (Golub & Van Loan,
+    # Matrix Computations, 4th ed., sec. 7.5.6), x4 for complex arithmetic.
+    f_qr = 4.0 * (25.0 - 14.0 / 3.0) * n ** 3
+    launches = max(st["n_eig_launches"], 1)
+    per_launch_mats = st["n_eig_matrices"] / launches
+    avg_ms = st["ms_eig_schur"] / launches
+    achieved = f_qr * per_launch_mats / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    roofline = {"kernel": "schur_kernel (complex single-shift QR sweeps, T and Z)",
+                "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "avg_launch_ms": avg_ms, "matrices_per_launch": per_launch_mats}
+    whole = st["flops_algorithmic"] / (st["ms_total"] * 1e-3) / 1e12 if st["ms_total"] else 0.0
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "21x21 harmonics (N=441, 882x882 complex128), 20 layers, "
+                               "512x512 grids, lambda 250-800 nm sweep, theta 65 deg, TE+TM",
+                   "lams_per_step": args.lams, "solves_timed": solves},
+        "wavelength_points_per_s": value,
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
+                       "p99": float(np.percentile(step_ms, 99)),
+                       "what": "per-step wall time; every solve of a step finishes with it"},
+        "path_fp64_tflops": whole,
+        "path_fp64_frac": whole / FP64_PEAK_TFLOPS,
+        "stage_ms": {k: st[k] for k in ("ms_convmat", "ms_layers", "ms_eig", "ms_eig_hess",
+                                        "ms_eig_schur", "ms_eig_vec", "ms_fold", "ms_out")},
+        "roofline": roofline,
+    }
+
+    if args.split:
+        sres = []
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            sres.append(run(args.warmup + k, G.RCWA_SPLIT))
+        hip.sync()
+        selapsed = time.perf_counter() - t0
+        dR = max(np.abs(a[0] - b[0]).max() for ra, rb in zip(results, sres) for a, b in zip(ra, rb))
+        dT = max(np.abs(a[1] - b[1]).max() for ra, rb in zip(results, sres) for a, b in zip(ra, rb))
+        out["split"] = {"solves_per_s": solves / selapsed, "speedup_vs_fp64": elapsed / selapsed,
+                        "max_dR_vs_fp64": float(dR), "max_dT_vs_fp64": float(dT)}
+
+    if cpu_procs:
+        t = {}
+        for pr in cpu_procs:
+            o, e = pr.communicate(timeout=1800)
+            if pr.returncode == 0:
+                d = json.loads(o.strip().splitlines()[-1])
+                t[d["which"]] = d["seconds"]
+        if len(t) == 2:
+            per_solve = 20 * t["layer"] + 22 * t["star"]
+            out["cpu_baseline"] = {
+                "value": 1.0 / per_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+                "sample": "CPU FP64 oracle, one thread: one SMatrixLayer (%.1f s) and one "
+                          "RedhefferProduct (%.1f s) at N=441, extrapolated to 20 layers + "
+                          "22 star products per solve" % (t["layer"], t["star"])}
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

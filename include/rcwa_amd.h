@@ -93,6 +93,10 @@ typedef struct {
   double ms_total, ms_convmat, ms_layers, ms_eig, ms_fold, ms_out;
   double flops_algorithmic; /* 2384 N^3 per layer + 1621 N^3 per S-matrix */
   int n_smatrices, n_layers;
+  /* eigensolver phases, filled only when rcwa_ctx_set_timing(ctx, 1): the
+   * stream is synchronised after each layer batch to read the events */
+  double ms_eig_hess, ms_eig_schur, ms_eig_vec;
+  int n_eig_launches, n_eig_matrices;
 } rcwa_stats;
 
 typedef struct rcwa_ctx rcwa_ctx;
@@ -111,6 +115,8 @@ rcwa_status rcwa_ctx_reset_stats(rcwa_ctx* ctx);
 /* Upper bound on the number of S-matrices batched into one chunk (0 = as
  * many as the HBM budget allows). */
 rcwa_status rcwa_ctx_set_chunk(rcwa_ctx* ctx, int max_chunk);
+/* Per-phase eigensolver timing with HIP events (adds a synchronisation). */
+rcwa_status rcwa_ctx_set_timing(rcwa_ctx* ctx, int on);
 
 /* Full solve, replacing the main.cpp:126-191 sequence (ConvMat per layer ->
  * ComputeWaveVectors -> GapMedium -> SMatrixInit -> per-layer SMatrixLayer

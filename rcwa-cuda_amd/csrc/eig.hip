@@ -230,6 +230,185 @@ __global__ __launch_bounds__(QR_THREADS) void schur_kernel(int n, ZMat Tm, ZMat 
   }
 }
 
+// Windowed variant of the QR sweep.  A sweep over the active block [il, iu]
+// is cut into windows of R consecutive Givens rotations.  Per window:
+//   1. the diagonal band rows/cols [k0-1, k0+R+1] of T is staged in LDS;
+//   2. wave 0 chases the bulge through the band: it computes the R
+//      rotations and applies each one to the band only (rows, then columns,
+//      in the sequential order of the unwindowed sweep);
+//   3. all threads write the band back and apply the R rotations, held in
+//      LDS, to everything outside the band: the rows [k0, k0+R] right of the
+//      band (one thread per column, its R+1 entries in registers), the
+//      columns [k0, k0+R] above the band and the same columns of Z (one
+//      thread per row).  Each element outside the band is read and written
+//      once per window instead of once per rotation.
+// The arithmetic per element and the rotation order are those of the
+// unwindowed sweep.
+template <int R>
+__global__ __launch_bounds__(QR_THREADS) void schur_win_kernel(int n, ZMat Tm, ZMat Zm, int* info,
+                                                               int maxit_per_row) {
+  constexpr int LDB = R + 3;
+  __shared__ zc band[LDB * LDB];
+  __shared__ double rc[R];
+  __shared__ zc rs[R];
+  __shared__ int s_il;
+  __shared__ zc s_mu;
+  const int b = blockIdx.x;
+  zc* T = Tm.p + (long long)b * Tm.stride;
+  zc* Z = Zm.p + (long long)b * Zm.stride;
+  const long long ld = Tm.ld, ldz = Zm.ld;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double eps = DBL_EPSILON;
+  const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
+  int iu = n - 1, iter = 0;
+  long long total = 0;
+  while (iu > 0) {
+    if (tid == 0) s_il = 0;
+    __syncthreads();
+    for (int k = iu - tid; k >= 1; k -= QR_THREADS) {
+      const double sub = zabs1(T[k + (k - 1) * ld]);
+      const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
+      if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
+    }
+    __syncthreads();
+    const int il = s_il;
+    if (il == iu) {
+      if (tid == 0) T[iu + (iu - 1) * ld] = zmake(0, 0);
+      --iu;
+      iter = 0;
+      __syncthreads();
+      continue;
+    }
+    ++iter;
+    ++total;
+    if (total > maxit) {
+      if (tid == 0 && info) info[b] = 1;
+      break;
+    }
+    if (tid == 0) {
+      if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
+      zc mu;
+      if (iter == 10)
+        mu = zadd(T[il + il * ld], zmake(0.75 * fabs(T[(il + 1) + il * ld].x), 0));
+      else if (iter == 20)
+        mu = zadd(T[iu + iu * ld], zmake(0.75 * fabs(T[iu + (iu - 1) * ld].x), 0));
+      else
+        mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
+                       T[iu + (iu - 1) * ld], T[iu + iu * ld]);
+      s_mu = mu;
+    }
+    __syncthreads();
+    for (int k0 = il; k0 < iu;) {
+      const int k1 = min(k0 + R, iu);          // rotations k0 .. k1-1
+      const int nr = k1 - k0;
+      const int b0 = k0 > 0 ? k0 - 1 : 0;      // band rows/cols [b0, be]
+      const int be = min(k1 + 1, n - 1);
+      const int nb = be - b0 + 1;
+      for (int idx = tid; idx < nb * nb; idx += QR_THREADS) {
+        const int i = idx % nb, j = idx / nb;
+        band[i + j * LDB] = T[(b0 + i) + (long long)(b0 + j) * ld];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        for (int k = k0; k < k1; ++k) {
+          double c = 0.0;
+          zc sr = zmake(0, 0);
+          if (lane == 0) {
+            zc f, g, r;
+            if (k == il) {
+              f = zsub(band[(k - b0) + (k - b0) * LDB], s_mu);
+              g = band[(k + 1 - b0) + (k - b0) * LDB];
+            } else {
+              f = band[(k - b0) + (k - 1 - b0) * LDB];
+              g = band[(k + 1 - b0) + (k - 1 - b0) * LDB];
+            }
+            givens(f, g, c, sr, r);
+            if (k > il) {
+              band[(k - b0) + (k - 1 - b0) * LDB] = r;
+              band[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
+            }
+            rc[k - k0] = c;
+            rs[k - k0] = sr;
+          }
+          c = __shfl(c, 0);
+          sr = zmake(__shfl(sr.x, 0), __shfl(sr.y, 0));
+          const zc sc = zconj(sr);
+          // rows k, k+1 over band columns k .. be
+          for (int j = k + lane; j <= be; j += 64) {
+            zc* p0 = &band[(k - b0) + (j - b0) * LDB];
+            const zc x = p0[0], y = p0[1];
+            p0[0] = zadd(zscale(x, c), zmul(sr, y));
+            p0[1] = zsub(zscale(y, c), zmul(sc, x));
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          // columns k, k+1 over band rows b0 .. min(k+2, iu)
+          const int ie = min(k + 2, iu);
+          for (int i = b0 + lane; i <= ie; i += 64) {
+            zc* q0 = &band[(i - b0) + (k - b0) * LDB];
+            const zc x = q0[0], y = q0[LDB];
+            q0[0] = zadd(zscale(x, c), zmul(sc, y));
+            q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __syncthreads();
+      // band back to T
+      for (int idx = tid; idx < nb * nb; idx += QR_THREADS) {
+        const int i = idx % nb, j = idx / nb;
+        T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
+      }
+      // rows k0..k1 right of the band: one column per thread
+      for (int j = be + 1 + tid; j < n; j += QR_THREADS) {
+        zc v[R + 1];
+        zc* col = T + k0 + (long long)j * ld;
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) v[q] = col[q];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (q < nr) {
+            const double c = rc[q];
+            const zc sr = rs[q], sc = zconj(sr);
+            const zc x = v[q], y = v[q + 1];
+            v[q] = zadd(zscale(x, c), zmul(sr, y));
+            v[q + 1] = zsub(zscale(y, c), zmul(sc, x));
+          }
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) col[q] = v[q];
+      }
+      // columns k0..k1 above the band (rows 0..b0-1 of T) and of Z (all rows)
+      for (int i = tid; i < b0 + n; i += QR_THREADS) {
+        zc* base;
+        long long lds;
+        if (i < b0) { base = T + i + (long long)k0 * ld; lds = ld; }
+        else { base = Z + (i - b0) + (long long)k0 * ldz; lds = ldz; }
+        zc v[R + 1];
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) v[q] = base[q * lds];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (q < nr) {
+            const double c = rc[q];
+            const zc sr = rs[q], sc = zconj(sr);
+            const zc x = v[q], y = v[q + 1];
+            v[q] = zadd(zscale(x, c), zmul(sc, y));
+            v[q + 1] = zsub(zscale(y, c), zmul(sr, x));
+          }
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) base[q * lds] = v[q];
+      }
+      __syncthreads();
+      k0 = k1;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void trevc_kernel(int n, ZCMat Tm, ZMat Xm) {
   extern __shared__ zc rhs[];
   const int k = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
@@ -277,6 +456,14 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch) {
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                   int maxit_per_row) {
   if (n <= 0 || batch <= 0) return 0;
+  hipLaunchKernelGGL(schur_win_kernel<12>, dim3(batch), dim3(QR_THREADS), 0, s, n, T, Z, info,
+                     maxit_per_row);
+  return (int)hipGetLastError();
+}
+
+int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
+                             int maxit_per_row) {
+  if (n <= 0 || batch <= 0) return 0;
   hipLaunchKernelGGL(schur_kernel, dim3(batch), dim3(QR_THREADS), 0, s, n, T, Z, info,
                      maxit_per_row);
   return (int)hipGetLastError();
@@ -293,14 +480,17 @@ size_t eig_workspace_bytes(int n, int batch) {
 }
 
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row) {
+                int maxit_per_row, hipEvent_t* marks) {
   if (n <= 0 || batch <= 0) return 0;
   zc* Zp = (zc*)ws;
   zc* Xp = Zp + (size_t)batch * n * n;
   ZMat Z{Zp, n, (long long)n * n}, X{Xp, n, (long long)n * n};
   int e;
+  if (marks) (void)hipEventRecord(marks[0], s);
   if ((e = hessenberg_batched(s, n, A, Z, batch))) return e;
+  if (marks) (void)hipEventRecord(marks[1], s);
   if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row))) return e;
+  if (marks) (void)hipEventRecord(marks[2], s);
   if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch))) return e;
   hipLaunchKernelGGL(diag_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n,
                      ZCMat{A.p, A.ld, A.stride}, w);
@@ -308,5 +498,7 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
                          ZCMat{Xp, n, (long long)n * n}, zmake(0, 0), ZCMat{nullptr, 0, 0}, W,
                          batch)))
     return e;
-  return znormalize_cols(s, n, W, batch);
+  if ((e = znormalize_cols(s, n, W, batch))) return e;
+  if (marks) (void)hipEventRecord(marks[3], s);
+  return 0;
 }

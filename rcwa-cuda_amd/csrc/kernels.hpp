@@ -36,11 +36,16 @@ int zcount_nonfinite(hipStream_t s, const zc* p, long long count, int* flag);
 // info[b] = 1 if the QR iteration did not converge within maxit_per_row*n
 // sweeps.
 size_t eig_workspace_bytes(int n, int batch);
+// marks (optional): four events recorded before the Hessenberg reduction,
+// before the QR sweeps, before the eigenvector phase and at the end.
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row);
+                int maxit_per_row, hipEvent_t* marks = nullptr);
 // The three phases, exposed for tests.
 int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row);
+// The same sweeps with every rotation applied directly in global memory.
+int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
+                             int maxit_per_row);
 int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch);
 
 // rcwa_kernels.hip -----------------------------------------------------------

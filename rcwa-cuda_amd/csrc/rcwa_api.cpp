@@ -144,9 +144,21 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
               Bl));
   CK(c.gemm(n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
   // eigendecomposition (rcwa.cpp:298-306)
-  if (c.ev_on) CKH(hipEventRecord(c.ev[2], s));
-  CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit));
-  if (c.ev_on) CKH(hipEventRecord(c.ev[3], s));
+  CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
+                 c.time_eig ? c.ev + 8 : nullptr));
+  if (c.time_eig) {
+    float th = 0, ts = 0, tv = 0;
+    CKH(hipEventSynchronize(c.ev[11]));
+    CKH(hipEventElapsedTime(&th, c.ev[8], c.ev[9]));
+    CKH(hipEventElapsedTime(&ts, c.ev[9], c.ev[10]));
+    CKH(hipEventElapsedTime(&tv, c.ev[10], c.ev[11]));
+    c.stats.ms_eig += th + ts + tv;
+    c.stats.ms_eig_hess += th;
+    c.stats.ms_eig_schur += ts;
+    c.stats.ms_eig_vec += tv;
+    c.stats.n_eig_launches += 1;
+    c.stats.n_eig_matrices += Bl;
+  }
   // lambda, X (rcwa.cpp:317, 339); V = Q W diag(1/lambda) (rcwa.cpp:326-330)
   CK(layer_modes(s, n, lam2, lam, ilam, X, k0t, Bl));
   CK(zscale_cols_to(s, n, n, CM(b3), ilam, n, M(b0), Bl));
@@ -413,7 +425,6 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   }
   CKH(hipEventRecord(c.ev[1], s));
   // ---- SMatrixLayer for every layer (main.cpp:151)
-  float ms_eig = 0;
   for (int l0 = 0; l0 < TL; l0 += Bl) {
     const int b = std::min(Bl, TL - l0);
     st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
@@ -423,12 +434,6 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
                      ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
                      info_eig + l0, info_sing + l0);
     if (st) return st;
-    if (c.time_eig) {
-      CKH(hipEventSynchronize(c.ev[3]));
-      float t = 0;
-      CKH(hipEventElapsedTime(&t, c.ev[2], c.ev[3]));
-      ms_eig += t;
-    }
   }
   CKH(hipEventRecord(c.ev[4], s));
   // ---- fold (main.cpp:143-164), ends (main.cpp:166-170)
@@ -490,7 +495,6 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   CKH(hipEventElapsedTime(&t56, c.ev[5], c.ev[6]));
   c.stats.ms_convmat += t01;
   c.stats.ms_layers += t14;
-  c.stats.ms_eig += ms_eig;
   c.stats.ms_fold += t45;
   c.stats.ms_out += t56;
   c.stats.ms_total += t01 + t14 + t45 + t56;
@@ -644,6 +648,12 @@ rcwa_status rcwa_ctx_set_eig_maxit(rcwa_ctx* ctx, int maxit) {
 rcwa_status rcwa_ctx_set_chunk(rcwa_ctx* ctx, int max_chunk) {
   if (!ctx) return rcwa_set_error(RCWA_E_ARG, "null ctx");
   reinterpret_cast<Ctx*>(ctx)->max_chunk = max_chunk;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_set_timing(rcwa_ctx* ctx, int on) {
+  if (!ctx) return rcwa_set_error(RCWA_E_ARG, "null ctx");
+  reinterpret_cast<Ctx*>(ctx)->time_eig = on != 0;
   return RCWA_OK;
 }
 

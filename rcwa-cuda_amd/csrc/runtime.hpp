@@ -26,7 +26,7 @@ struct Ctx {
   rcwa_precision precision = RCWA_FP64;
   rcwa_stats stats{};
   std::vector<std::complex<double>> last_cols;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[12] = {};
   bool ev_on = false;
   bool time_eig = false;  // synchronise after each layer sub-batch to time the eigensolver
 

@@ -64,7 +64,9 @@ class Stats(ctypes.Structure):
                 ("ms_layers", ctypes.c_double), ("ms_eig", ctypes.c_double),
                 ("ms_fold", ctypes.c_double), ("ms_out", ctypes.c_double),
                 ("flops_algorithmic", ctypes.c_double), ("n_smatrices", ctypes.c_int),
-                ("n_layers", ctypes.c_int)]
+                ("n_layers", ctypes.c_int), ("ms_eig_hess", ctypes.c_double),
+                ("ms_eig_schur", ctypes.c_double), ("ms_eig_vec", ctypes.c_double),
+                ("n_eig_launches", ctypes.c_int), ("n_eig_matrices", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -81,6 +83,7 @@ class TicketInfo(ctypes.Structure):
 EXPORTS = [
     "rcwa_last_error", "rcwa_version", "rcwa_ctx_create", "rcwa_ctx_destroy",
     "rcwa_ctx_set_eig_maxit", "rcwa_ctx_get_stats", "rcwa_ctx_reset_stats", "rcwa_ctx_set_chunk",
+    "rcwa_ctx_set_timing",
     "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
     "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
@@ -162,6 +165,9 @@ class Context:
     def set_eig_maxit(self, m):
         _check(lib().rcwa_ctx_set_eig_maxit(self.h, m))
 
+    def set_timing(self, on):
+        _check(lib().rcwa_ctx_set_timing(self.h, 1 if on else 0))
+
     def set_chunk(self, m):
         _check(lib().rcwa_ctx_set_chunk(self.h, m))
 
@@ -174,10 +180,11 @@ class Context:
         _check(lib().rcwa_ctx_reset_stats(self.h))
 
     # ---------------------------------------------------------------- solves
-    def solve(self, probs, precision=RCWA_FP64, device_grids=None):
+    def solve(self, probs, precision=RCWA_FP64, device_grids=None, grid_shape=None):
         """Solve a list of workloads.Problem.  Returns a list (per problem) of
         (R[npol][Q][P], T[npol][Q][P], tot[npol][2]).  device_grids, if given,
-        is a list of (er_dev_ptr, ur_dev_ptr) for HBM-resident grids."""
+        is a list of (er_dev_ptr, ur_dev_ptr) for HBM-resident grids of shape
+        [len(p.thick)][grid_shape[0]][grid_shape[1]]."""
         keep = []
         cprobs = (Problem * len(probs))()
         outs = []
@@ -196,8 +203,8 @@ class Context:
             else:
                 er_p, ur_p = p.er.ctypes.data, p.ur.ctypes.data
                 ondev = 0
-            d = Device(p.er.shape[2], p.er.shape[1], p.L, p.Lx, p.Ly, er_p, ur_p,
-                       p.thick.ctypes.data, ondev)
+            Ny, Nx = grid_shape if grid_shape is not None else p.er.shape[1:]
+            d = Device(Nx, Ny, len(p.thick), p.Lx, p.Ly, er_p, ur_p, p.thick.ctypes.data, ondev)
             par = Params(p.hx, p.hy, _c(p.er_ref), _c(p.ur_ref), _c(p.er_trn), _c(p.ur_trn))
             srcs = (Source * npol)(*[Source(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols])
             keep += [d, par, srcs, p]

@@ -108,18 +108,20 @@ def c2_profile(grid=512, period=120.0, cd=40.0, height=100.0, nslice=20, sidewal
 
 
 def c2_sweep(nlam=200, grid=512, h=10, nslice=20, lam_lo=250.0, lam_hi=800.0, seed=2,
-             theta_deg=65.0):
-    """List of Problems, one per wavelength, each carrying TE and TM."""
+             theta_deg=65.0, indices=None):
+    """Problems of the 200-wavelength sweep (or the given wavelength
+    indices), each carrying TE and TM.  All problems share one all-ones
+    permeability array."""
     lams = np.linspace(lam_lo, lam_hi, nlam)
     mats = c2_materials(lams, 5, seed)
     masks, thick = c2_profile(grid, nslice=nslice)
+    ur = np.ones((nslice, grid, grid), np.complex128)
     probs = []
-    for i, lam in enumerate(lams):
+    for i in (range(nlam) if indices is None else indices):
         er = np.ones((nslice, grid, grid), np.complex128)
         for k in range(nslice):
             er[k][:, masks[k]] = mats[(k * 5) // nslice, i]
-        ur = np.ones_like(er)
-        probs.append(Problem(er, ur, thick, 120.0, 120.0, lam, math.radians(theta_deg), 0.0,
+        probs.append(Problem(er, ur, thick, 120.0, 120.0, lams[i], math.radians(theta_deg), 0.0,
                              [(1.0, 0.0), (0.0, 1.0)], h, h, 1.0, 1.0, 2.25, 1.0,
                              name="c2"))
     return probs
