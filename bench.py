@@ -83,14 +83,58 @@ class Hip:
         self.lib.hipFree(ctypes.c_void_p(p))
 
 
+def serve_bench(args):
+    """Serving form on a prefix of the seeded Poisson stream (mean 2000
+    req/s, harmonics {7,11,15,21,25}^2, 1-30 layers, 256x256 grids): each
+    request is submitted at its arrival time; then the same requests run
+    one per call through the plain path."""
+    import threading
+    import rcwa_amd as G
+    import workloads as W
+    stream = W.c4_stream(count=args.serve)
+    with G.Server(0, n_workers=2) as srv:
+        tickets = [None] * len(stream)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, (ta, p) in enumerate(stream):
+                d = t0 + ta - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets[i] = srv.submit(p)
+        th = threading.Thread(target=submitter)
+        c0 = srv.clock()
+        th.start()
+        th.join()
+        res = [srv.wait(t) for t in tickets]
+    lat = np.array([r[3]["t_finish"] - r[3]["t_submit"] for r in res])
+    first = min(r[3]["t_submit"] for r in res)
+    last = max(r[3]["t_finish"] for r in res)
+    served = len(res) / (last - first)
+    ctx = G.Context(0)
+    t1 = time.perf_counter()
+    alone = [ctx.solve([p])[0] for _, p in stream]
+    percall = len(stream) / (time.perf_counter() - t1)
+    ctx.close()
+    dmax = max(float(np.abs(a[0] - r[0]).max()) for a, r in zip(alone, res))
+    return {"requests": len(res), "solves_per_s": served, "percall_solves_per_s": percall,
+            "speedup_vs_percall": served / percall,
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                           "p99": float(np.percentile(lat, 99) * 1e3)},
+            "mean_batch": float(np.mean([r[3]["batch_size"] for r in res])),
+            "max_dR_mix_vs_alone": dmax, "clock_start": c0}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lams", type=int, default=8, help="wavelengths per step")
+    ap.add_argument("--lams", type=int, default=12, help="wavelengths per step")
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="also run the serving form on this many requests of the stream")
     ap.add_argument("--cpu-sample", choices=["layer", "star"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_sample:
@@ -203,6 +247,9 @@ def main():
         dT = max(np.abs(a[1] - b[1]).max() for ra, rb in zip(results, sres) for a, b in zip(ra, rb))
         out["split"] = {"solves_per_s": solves / selapsed, "speedup_vs_fp64": elapsed / selapsed,
                         "max_dR_vs_fp64": float(dR), "max_dT_vs_fp64": float(dT)}
+
+    if args.serve:
+        out["serving"] = serve_bench(args)
 
     if cpu_procs:
         t = {}

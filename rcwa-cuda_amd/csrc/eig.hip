@@ -17,6 +17,7 @@
 // eigenvectors and not on their scale or order, so W need not match Eigen's
 // normalisation or sort order.
 #include <cfloat>
+#include <cstdlib>
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -244,9 +245,9 @@ __global__ __launch_bounds__(QR_THREADS) void schur_kernel(int n, ZMat Tm, ZMat 
 //      once per window instead of once per rotation.
 // The arithmetic per element and the rotation order are those of the
 // unwindowed sweep.
-template <int R>
-__global__ __launch_bounds__(QR_THREADS) void schur_win_kernel(int n, ZMat Tm, ZMat Zm, int* info,
-                                                               int maxit_per_row) {
+template <int R, int NT>
+__global__ __launch_bounds__(NT) void schur_win_kernel(int n, ZMat Tm, ZMat Zm, int* info,
+                                                       int maxit_per_row) {
   constexpr int LDB = R + 3;
   __shared__ zc band[LDB * LDB];
   __shared__ double rc[R];
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(QR_THREADS) void schur_win_kernel(int n, ZMat Tm, Z
   while (iu > 0) {
     if (tid == 0) s_il = 0;
     __syncthreads();
-    for (int k = iu - tid; k >= 1; k -= QR_THREADS) {
+    for (int k = iu - tid; k >= 1; k -= NT) {
       const double sub = zabs1(T[k + (k - 1) * ld]);
       const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
       if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(QR_THREADS) void schur_win_kernel(int n, ZMat Tm, Z
       const int b0 = k0 > 0 ? k0 - 1 : 0;      // band rows/cols [b0, be]
       const int be = min(k1 + 1, n - 1);
       const int nb = be - b0 + 1;
-      for (int idx = tid; idx < nb * nb; idx += QR_THREADS) {
+      for (int idx = tid; idx < nb * nb; idx += NT) {
         const int i = idx % nb, j = idx / nb;
         band[i + j * LDB] = T[(b0 + i) + (long long)(b0 + j) * ld];
       }
@@ -356,12 +357,12 @@ __global__ __launch_bounds__(QR_THREADS) void schur_win_kernel(int n, ZMat Tm, Z
       }
       __syncthreads();
       // band back to T
-      for (int idx = tid; idx < nb * nb; idx += QR_THREADS) {
+      for (int idx = tid; idx < nb * nb; idx += NT) {
         const int i = idx % nb, j = idx / nb;
         T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
       }
       // rows k0..k1 right of the band: one column per thread
-      for (int j = be + 1 + tid; j < n; j += QR_THREADS) {
+      for (int j = be + 1 + tid; j < n; j += NT) {
         zc v[R + 1];
         zc* col = T + k0 + (long long)j * ld;
 #pragma unroll
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(QR_THREADS) void schur_win_kernel(int n, ZMat Tm, Z
           if (q <= nr) col[q] = v[q];
       }
       // columns k0..k1 above the band (rows 0..b0-1 of T) and of Z (all rows)
-      for (int i = tid; i < b0 + n; i += QR_THREADS) {
+      for (int i = tid; i < b0 + n; i += NT) {
         zc* base;
         long long lds;
         if (i < b0) { base = T + i + (long long)k0 * ld; lds = ld; }
@@ -453,10 +454,19 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch) {
   return (int)hipGetLastError();
 }
 
+int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
+                             int maxit_per_row);
+
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                   int maxit_per_row) {
   if (n <= 0 || batch <= 0) return 0;
-  hipLaunchKernelGGL(schur_win_kernel<12>, dim3(batch), dim3(QR_THREADS), 0, s, n, T, Z, info,
+  // RCWA_QR_SWEEP=direct selects the unwindowed sweep (same arithmetic).
+  static const bool direct = [] {
+    const char* e = getenv("RCWA_QR_SWEEP");
+    return e && e[0] == 'd';
+  }();
+  if (direct) return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
+  hipLaunchKernelGGL((schur_win_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z, info,
                      maxit_per_row);
   return (int)hipGetLastError();
 }

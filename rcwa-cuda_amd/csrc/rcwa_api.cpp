@@ -73,6 +73,9 @@ rcwa_status Ctx::init(int dev, size_t bud) {
 }
 
 void Ctx::release() {
+  if (split_ws) (void)hipFree(split_ws);
+  split_ws = nullptr;
+  split_cap = 0;
   if (arena) (void)hipFree(arena);
   arena = nullptr;
   arena_cap = 0;
@@ -111,8 +114,19 @@ void* Ctx::take(size_t bytes) {
 // ------------------------------------------------------------------ GEMM dispatch
 int Ctx::gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C,
               int batch) {
-  if (precision == RCWA_SPLIT)
-    return zgemm_split_batched(stream, M, N, K, alpha, A, B, beta, D, C, batch);
+  if (precision == RCWA_SPLIT) {
+    const size_t need = zgemm_split_ws_bytes(M, N, K, batch);
+    if (need > split_cap) {
+      hipError_t e = hipStreamSynchronize(stream);
+      if (e == hipSuccess && split_ws) e = hipFree(split_ws);
+      split_ws = nullptr;
+      split_cap = 0;
+      if (e == hipSuccess) e = hipMalloc(&split_ws, need);
+      if (e != hipSuccess) return (int)e;
+      split_cap = need;
+    }
+    return zgemm_split_batched_ws(stream, M, N, K, alpha, A, B, beta, D, C, batch, split_ws);
+  }
   return zgemm_batched(stream, M, N, K, alpha, A, B, beta, D, C, batch);
 }
 
@@ -197,10 +211,12 @@ void LayerWork::carve(Ctx& c, int N, int Bl) {
 }
 
 // ------------------------------------------------------------------ star product
-// S = A * B (rcwa.cpp:588-615) for a batch.  A's four slots receive S; the
-// slot pointers in A and tmp may be permuted (no copies).
+// S = A * B (rcwa.cpp:588-615) for a batch.  A's four slots receive S.
+// Unless keep_slots is set, the slot pointers in A and tmp may be permuted
+// instead of copying results; keep_slots is needed when the batch covers only
+// a prefix of the slot buffers (the other entries must stay where they are).
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
-                         void* gjws, int* info_sing) {
+                         void* gjws, int* info_sing, bool keep_slots) {
   hipStream_t s = c.stream;
   const long long n2 = (long long)n * n;
   auto M = [&](zc* p) { return ZMat{p, n, n2}; };
@@ -223,11 +239,13 @@ rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], z
   // S12 = A12 (F1^-1 B12) (rcwa.cpp:609)
   CK(c.gemm(n, n, n, one, CM(tmp[0]), B12, zero, none, M(tmp[2]), batch));
   CK(c.gemm(n, n, n, one, CM(A12), CM(tmp[2]), zero, none, M(tmp[3]), batch));
-  std::swap(A12, tmp[3]);
+  if (keep_slots) CK(zcopy_batched(s, n, n, CM(tmp[3]), M(A12), batch));
+  else std::swap(A12, tmp[3]);
   // S21 = B21 (F2^-1 A21) (rcwa.cpp:610)
   CK(c.gemm(n, n, n, one, CM(tmp[1]), CM(A21), zero, none, M(tmp[2]), batch));
   CK(c.gemm(n, n, n, one, B21, CM(tmp[2]), zero, none, M(tmp[4]), batch));
-  std::swap(A21, tmp[4]);
+  if (keep_slots) CK(zcopy_batched(s, n, n, CM(tmp[4]), M(A21), batch));
+  else std::swap(A21, tmp[4]);
   // S22 = B22 + B21 (F2^-1 A22) B12 (rcwa.cpp:611)
   CK(c.gemm(n, n, n, one, CM(tmp[1]), CM(A22), zero, none, M(tmp[2]), batch));
   CK(c.gemm(n, n, n, one, B21, CM(tmp[2]), zero, none, M(tmp[4]), batch));
@@ -457,7 +475,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
       const zc* b11 = S11l + (size_t)off[l] * n2;
       const zc* b12 = S12l + (size_t)off[l] * n2;
       const ZCMat B[4] = {{b11, n, n2}, {b12, n, n2}, {b12, n, n2}, {b11, n, n2}};
-      st = star_product(c, n, act[l], S, B, T, gjf, info_sing);
+      st = star_product(c, n, act[l], S, B, T, gjf, info_sing, act[l] < G);
       if (st) return st;
     }
   }
@@ -466,7 +484,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     CK(expand_block_diag(s, N, dends + (size_t)4 * k * N, 32LL * N, ZMat{E[k], n, n2}, G));
   {
     const ZCMat B[4] = {{S[0], n, n2}, {S[1], n, n2}, {S[2], n, n2}, {S[3], n, n2}};
-    st = star_product(c, n, G, E, B, T, gjf, info_sing);
+    st = star_product(c, n, G, E, B, T, gjf, info_sing, false);
     if (st) return st;
   }
   for (int k = 0; k < 4; ++k)
@@ -474,7 +492,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
                          ZMat{S[k], n, n2}, G));
   {
     const ZCMat B[4] = {{S[0], n, n2}, {S[1], n, n2}, {S[2], n, n2}, {S[3], n, n2}};
-    st = star_product(c, n, G, E, B, T, gjf, info_sing);
+    st = star_product(c, n, G, E, B, T, gjf, info_sing, false);
     if (st) return st;
   }
   CKH(hipEventRecord(c.ev[5], s));

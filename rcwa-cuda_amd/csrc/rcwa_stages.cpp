@@ -247,7 +247,7 @@ rcwa_status rcwa_star_product(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* 
   CKH(hipMemsetAsync(dinfo, 0, batch * sizeof(int), c.stream));
   const ZCMat Bm[4] = {{b[0], n, (long long)n2}, {b[1], n, (long long)n2},
                        {b[2], n, (long long)n2}, {b[3], n, (long long)n2}};
-  if ((st = star_product(c, n, batch, a, Bm, t, gj, dinfo))) return st;
+  if ((st = star_product(c, n, batch, a, Bm, t, gj, dinfo, false))) return st;
   if ((st = check_info(c, dinfo, batch))) return st;
   for (int k = 0; k < 4; ++k)
     if ((st = d2h(c, S[k], a[k], blk))) return st;

@@ -9,10 +9,11 @@
 
 rcwa_status rcwa_set_error(rcwa_status s, const std::string& msg);
 
-// Split-precision complex GEMM (FP32 matrix cores, FP32 accumulation);
-// defined in split_gemm.hip.
-int zgemm_split_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
-                        ZCMat D, ZMat C, int batch);
+// Split-precision complex GEMM (FP16 slices on the matrix cores, FP32
+// accumulation); defined in split_gemm.hip.  ws holds zgemm_split_ws_bytes().
+size_t zgemm_split_ws_bytes(int M, int N, int K, int batch);
+int zgemm_split_batched_ws(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B,
+                           zc beta, ZCMat D, ZMat C, int batch, void* ws);
 
 // A solve context: one HIP stream, an HBM arena bounded by `budget`.
 struct Ctx {
@@ -20,6 +21,8 @@ struct Ctx {
   hipStream_t stream = nullptr;
   size_t budget = 0;
   char* arena = nullptr;
+  void* split_ws = nullptr;  // operand slices of the split-precision GEMM
+  size_t split_cap = 0;
   size_t arena_cap = 0, arena_off = 0;
   int eig_maxit = 30;
   int max_chunk = 0;
@@ -56,7 +59,7 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing);
 
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
-                         void* gjws, int* info_sing);
+                         void* gjws, int* info_sing, bool keep_slots);
 
 // A problem with the place its results go.
 struct ProblemRef {

@@ -13,6 +13,10 @@
 // real part to two MFMAs and the imaginary part to the other two
 // (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), so no deinterleaving is needed.
 //
+// Lane layout of v_mfma_f64_16x16x4f64 (measured on gfx950 with an
+// in-bounds probe): lane l supplies A[l%16][l/16] and B[l/16][l%16]; result
+// register r of lane l holds D[4r + l/16][l%16].
+//
 // Tiling: 256 threads = 4 wave64s in a 2x2 arrangement; the workgroup tile
 // is 64x64 of C, each wave owns 32x32 = 2x2 MFMA tiles of 16x16.  K advances
 // in slabs of 16 staged through LDS (double-buffered: the next slab's global
@@ -92,13 +96,19 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alph
       for (int i = 0; i < 2; ++i) a[i] = As[cur][kk + lk][wm * 32 + i * 16 + li];
 #pragma unroll
       for (int j = 0; j < 2; ++j) bb[j] = Bs[cur][kk + lk][wn * 32 + j * 16 + li];
+      // eight independent accumulators between two updates of the same one
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           accR[i][j] = mfma(a[i].x, bb[j].x, accR[i][j]);
-          accR[i][j] = mfma(-a[i].y, bb[j].y, accR[i][j]);
           accI[i][j] = mfma(a[i].x, bb[j].y, accI[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = mfma(-a[i].y, bb[j].y, accR[i][j]);
           accI[i][j] = mfma(a[i].y, bb[j].x, accI[i][j]);
         }
     }
@@ -119,7 +129,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alph
       if (col >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + i * 16 + 4 * lk + r;
+        const int row = m0 + wm * 32 + i * 16 + 4 * r + lk;  // D[4r + l/16][l%16]
         if (row >= M) continue;
         zc v = zmul(alpha, zmake(accR[i][j][r], accI[i][j][r]));
         if (use_d) v = zfma(beta, Dp[row + (long long)col * D.ld], v);

@@ -322,3 +322,68 @@ def end_smatrix(which, hx, hy, Kx, Ky, Kz, er_ref, ur_ref, er_trn, ur_trn):
     _check(lib().rcwa_end_smatrix(which, PQ, ctypes.byref(par), _ptr(_cz(Kx)), _ptr(_cz(Ky)),
                                   _ptr(_cz(Kz)), *[_ptr(x) for x in S]))
     return tuple(x.reshape(n, n).T for x in S)
+
+
+class Server:
+    """The serving form (rcwa_server): continuous batching of submitted
+    solves on one GPU."""
+
+    def __init__(self, device=0, hbm_budget=0, max_batch=0, n_workers=0,
+                 precision=RCWA_FP64):
+        self.h = ctypes.c_void_p()
+        cfg = ServerConfig(hbm_budget, max_batch, n_workers, precision)
+        _check(lib().rcwa_server_create(device, ctypes.byref(cfg), ctypes.byref(self.h)))
+        self._meta = {}
+
+    def submit(self, p):
+        L, Ny, Nx = p.er.shape
+        d = Device(Nx, Ny, L, p.Lx, p.Ly, p.er.ctypes.data, p.ur.ctypes.data,
+                   p.thick.ctypes.data, 0)
+        par = Params(p.hx, p.hy, _c(p.er_ref), _c(p.ur_ref), _c(p.er_trn), _c(p.ur_trn))
+        npol = len(p.pols)
+        srcs = (Source * npol)(*[Source(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols])
+        prob = Problem(ctypes.pointer(d), ctypes.pointer(par), srcs, npol)
+        t = ctypes.c_uint64()
+        _check(lib().rcwa_server_submit(self.h, ctypes.byref(prob), ctypes.byref(t)))
+        self._meta[t.value] = (p.hx, p.hy, npol)
+        return t.value
+
+    def _collect(self, fn, ticket):
+        hx, hy, npol = self._meta[ticket]
+        P, Q = 2 * hx + 1, 2 * hy + 1
+        R = np.zeros((npol, P * Q))
+        T = np.zeros((npol, P * Q))
+        res = (Results * npol)(*[Results(R[j].ctypes.data, T[j].ctypes.data, 0.0, 0.0)
+                                 for j in range(npol)])
+        info = TicketInfo()
+        rc = fn(self.h, ctypes.c_uint64(ticket), res, ctypes.byref(info))
+        if rc == 7:
+            return None
+        _check(rc)
+        tot = np.array([[res[j].R_tot, res[j].T_tot] for j in range(npol)])
+        return (R.reshape(npol, P, Q).transpose(0, 2, 1), T.reshape(npol, P, Q).transpose(0, 2, 1),
+                tot, {"t_submit": info.t_submit, "t_start": info.t_start,
+                      "t_finish": info.t_finish, "batch_size": info.batch_size})
+
+    def wait(self, ticket):
+        return self._collect(lib().rcwa_server_wait, ticket)
+
+    def poll(self, ticket):
+        return self._collect(lib().rcwa_server_poll, ticket)
+
+    def cancel(self, ticket):
+        _check(lib().rcwa_server_cancel(self.h, ctypes.c_uint64(ticket)))
+
+    def clock(self):
+        return lib().rcwa_server_clock(self.h)
+
+    def close(self):
+        if self.h:
+            lib().rcwa_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

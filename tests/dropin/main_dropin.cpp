@@ -9,7 +9,17 @@
 #include <iostream>
 #include "rcwa_amd.hpp"
 
+static int run();
 int main() {
+  try {
+    return run();
+  } catch (const rcwa_amd::Error& e) {
+    std::fprintf(stderr, "rcwa_amd::Error status %d: %s\n", (int)e.status, e.what());
+    return 3;
+  }
+}
+
+static int run() {
   int Nx, Ny, L, hx, hy;
   double Lx, Ly, lam, th, ph, pte, ptm;
   double m[8];
@@ -20,6 +30,10 @@ int main() {
   std::vector<Complex> er((size_t)L * Nx * Ny), ur((size_t)L * Nx * Ny);
   for (auto& v : er) { double a, b; std::cin >> a >> b; v = Complex(a, b); }
   for (auto& v : ur) { double a, b; std::cin >> a >> b; v = Complex(a, b); }
+  if (!std::cin) {
+    std::fprintf(stderr, "malformed input\n");
+    return 2;
+  }
 
   Device device(Nx, Ny, L, Lx, Ly, er, ur, thickness, hx, hy);
   Source source(lam, th, ph, pte, ptm);
@@ -49,6 +63,7 @@ int main() {
     S_layer = SMatrixLayer(layer, device, source, params, Kx, Ky, W0, V0);
     S_device = RedhefferProduct(S_device, S_layer);
   }
+  std::fprintf(stderr, "layers folded\n");
   ScatteringMatrix S_ref = SMatrixReflection(params, Kx, Ky, Kz_ref, W0, V0, W_ref);
   ScatteringMatrix S_trn = SMatrixTransmission(params, Kx, Ky, Kz_trn, W0, V0, W_trn);
   ScatteringMatrix S_global = RedhefferProduct(S_ref, S_device);
@@ -60,6 +75,7 @@ int main() {
   Vector t = Vector::Zero(3 * PQ);
   ComputeTransmittedField(params, S_global, csrc, Kx, Ky, Kz_trn, W_trn, t);
   Results results = ComputeDiffractionEfficiencies(params, r, t, k_inc, Kz_ref, Kz_trn);
+  std::fprintf(stderr, "staged path done\n");
   Results one = Solve(device, source, params);
   std::printf("%d\n", PQ);
   for (int i = 0; i < PQ; ++i) std::printf("%.17g %.17g\n", results.R(i), results.T(i));

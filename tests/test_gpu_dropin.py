@@ -16,14 +16,18 @@ EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "main_d
 
 def _stdin(p, pol):
     L, Ny, Nx = p.er.shape
-    hdr = [Nx, Ny, L, p.Lx, p.Ly, p.hx, p.hy, p.lam0, p.theta, p.phi, *p.pols[pol]]
     med = []
     for z in (p.er_ref, p.ur_ref, p.er_trn, p.ur_trn):
         med += [z.real, z.imag]
-    body = [repr(float(v)) for v in hdr + med] + [repr(float(t)) for t in p.thick]
+    ints = [Nx, Ny, L]
+    flts = [p.Lx, p.Ly]
+    ints2 = [p.hx, p.hy]
+    flts2 = [p.lam0, p.theta, p.phi, *p.pols[pol]] + med
+    body = ([str(v) for v in ints] + [repr(float(v)) for v in flts] + [str(v) for v in ints2]
+            + [repr(float(v)) for v in flts2] + [repr(float(t)) for t in p.thick])
     for g in (p.er, p.ur):
         f = g.reshape(-1)
-        body += ["%r %r" % (v.real, v.imag) for v in f]
+        body += ["%r %r" % (float(v.real), float(v.imag)) for v in f]
     return "\n".join(body) + "\n"
 
 

@@ -85,3 +85,18 @@ def test_convmat_vs_oracle(ctx, grid, hx, hy):
         c = ctx.convmat(g, layer, hx, hy)
         ref = O.convmat(g[layer], hx, hy)
         assert np.abs(c - ref).max() < 1e-14 * np.abs(g).max()
+
+
+@pytest.mark.parametrize("m,n,k", [(16, 16, 16), (70, 33, 100), (242, 242, 242)])
+def test_zgemm_split_accuracy(ctx, m, n, k):
+    """FP16-sliced products with FP32 accumulation: normwise error near
+    2^-22 + sqrt(k) 2^-24."""
+    rng = np.random.default_rng(m + n + k)
+    A = [rnd(rng, m, k) * np.exp(rng.uniform(-3, 3, (m, 1))) for _ in range(2)]
+    B = [rnd(rng, k, n) for _ in range(2)]
+    C = ctx.zgemm(A, B, split=True)
+    for a, b, c in zip(A, B, C):
+        ref = a @ b
+        bound = np.abs(a).max(axis=1, keepdims=True) * np.abs(b).max(axis=0, keepdims=True) * k
+        err = (np.abs(c - ref) / bound).max()
+        assert err < 2e-6, err

@@ -98,3 +98,29 @@ def test_c1_lossless_energy(ctx):
     assert abs(gpu[2][0].sum() - 1.0) < 1e-10
     ref = O.solve(p)
     assert max(_cmp(gpu, ref)) < TOL
+
+
+def test_split_form_error_vs_fp64(ctx):
+    """Split-precision form (FP16-sliced GEMMs, FP64 eigensolver and
+    inverses) against the FP64 form on the same structure; the error is
+    reported, and bounded by the 1e-6 target."""
+    p = W.c1_single(grid=128, h=3)
+    f = ctx.solve([p])[0]
+    s = ctx.solve([p], precision=G.RCWA_SPLIT)[0]
+    dR = np.abs(f[0] - s[0]).max()
+    dT = np.abs(f[1] - s[1]).max()
+    print("split vs fp64: max |dR| = %.3e, max |dT| = %.3e" % (dR, dT))
+    assert dR < 1e-6 and dT < 1e-6
+
+
+def test_solve_batch_mixed_layer_counts(ctx):
+    """One chunk with different layer counts: the fold runs on a shrinking
+    prefix of the jobs (layer-major layout)."""
+    rng = np.random.default_rng(31)
+    probs = [W.random_small(rng, hx=1, hy=2, L=L, grid=8) for L in (1, 4, 2, 3, 0, 4)]
+    probs[4].er = probs[4].er[:0]
+    probs[4].ur = probs[4].ur[:0]
+    probs[4].thick = probs[4].thick[:0]
+    out = ctx.solve(probs)
+    for p, g in zip(probs, out):
+        assert max(_cmp(g, O.solve(p))) < TOL

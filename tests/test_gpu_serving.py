@@ -1,0 +1,62 @@
+"""Serving form: requests of different harmonic counts, layer counts and
+wavelengths, submitted while others run, are batched together; each result
+is the same alone (plain per-call path) and in the mix."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import rcwa_amd as G
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(seed, count):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        h = int(rng.choice([1, 2, 3]))
+        L = int(rng.integers(1, 5))
+        p = W.random_small(rng, hx=h, hy=h, L=L, grid=16)
+        out.append(p)
+    return out
+
+
+def test_mixed_stream_same_alone_and_in_mix(ctx):
+    probs = _mix(21, 24)
+    with G.Server(0, hbm_budget=8 << 30, n_workers=2) as srv:
+        tickets = []
+        for i, p in enumerate(probs):
+            tickets.append(srv.submit(p))
+            if i % 6 == 5:
+                time.sleep(0.02)          # later requests arrive while earlier ones run
+        res = [srv.wait(t) for t in tickets]
+    batched = [r[3]["batch_size"] for r in res]
+    assert max(batched) > 1, "no request was batched with another"
+    for p, r in zip(probs, res):
+        alone = ctx.solve([p])[0]
+        assert np.abs(r[0] - alone[0]).max() < 1e-10
+        assert np.abs(r[1] - alone[1]).max() < 1e-10
+        assert r[3]["t_finish"] >= r[3]["t_start"] >= r[3]["t_submit"]
+    for p, r in list(zip(probs, res))[:3]:
+        R, T, tot = O.solve(p)
+        assert np.abs(r[0] - R).max() < 1e-9 and np.abs(r[1] - T).max() < 1e-9
+
+
+def test_cancel_and_poll():
+    probs = _mix(5, 6)
+    with G.Server(0, hbm_budget=4 << 30, n_workers=1, max_batch=1) as srv:
+        ts = [srv.submit(p) for p in probs]
+        cancelled = 0
+        try:
+            srv.cancel(ts[-1])
+            cancelled = 1
+        except G.RcwaError:
+            pass                                 # already started: nothing to cancel
+        for t in ts[:-1]:
+            assert srv.wait(t) is not None
+        if cancelled:
+            with pytest.raises(G.RcwaError):
+                srv.poll(ts[-1])
