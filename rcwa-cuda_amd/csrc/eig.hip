@@ -486,7 +486,7 @@ int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch) {
 }
 
 size_t eig_workspace_bytes(int n, int batch) {
-  return 2 * (size_t)batch * n * n * sizeof(zc) + 256;
+  return 2 * (size_t)batch * n * n * sizeof(zc) + hess_blocked_ws_bytes(n, batch) + 512;
 }
 
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
@@ -497,7 +497,17 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   ZMat Z{Zp, n, (long long)n * n}, X{Xp, n, (long long)n * n};
   int e;
   if (marks) (void)hipEventRecord(marks[0], s);
-  if ((e = hessenberg_batched(s, n, A, Z, batch))) return e;
+  // RCWA_HESS=unblocked selects the one-workgroup unblocked reduction.
+  static const bool unblocked = [] {
+    const char* v = getenv("RCWA_HESS");
+    return v && v[0] == 'u';
+  }();
+  if (unblocked || n > 4096) {
+    if ((e = hessenberg_batched(s, n, A, Z, batch))) return e;
+  } else {
+    void* hws = (void*)(((uintptr_t)(Xp + (size_t)batch * n * n) + 255) & ~(uintptr_t)255);
+    if ((e = hessenberg_blocked_batched(s, n, A, Z, batch, hws))) return e;
+  }
   if (marks) (void)hipEventRecord(marks[1], s);
   if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row))) return e;
   if (marks) (void)hipEventRecord(marks[2], s);

@@ -42,6 +42,9 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
                 int maxit_per_row, hipEvent_t* marks = nullptr);
 // The three phases, exposed for tests.
 int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
+// Blocked (compact-WY panels + ZGEMM updates); ws holds hess_blocked_ws_bytes(n, batch).
+size_t hess_blocked_ws_bytes(int n, int batch);
+int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, void* ws);
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row);
 // The same sweeps with every rotation applied directly in global memory.
 int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
