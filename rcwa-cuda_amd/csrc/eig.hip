@@ -25,6 +25,7 @@ namespace {
 
 constexpr int HESS_THREADS = 1024;
 constexpr int QR_THREADS = 512;
+constexpr int SCHUR_G = 8;  // sweeps per deferred-update pass
 
 __device__ __forceinline__ zc shfl_down_z(zc v, int off) {
   return zmake(__shfl_down(v.x, off), __shfl_down(v.y, off));
@@ -410,6 +411,325 @@ __global__ __launch_bounds__(NT) void schur_win_kernel(int n, ZMat Tm, ZMat Zm, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// QR sweeps with deferred updates.  The iteration itself only ever reads the
+// active block T[il:iu+1, il:iu+1], so each sweep updates just that block
+// (windowed as in schur_win_kernel) and logs its rotations.  Everything else
+// the sweep changes -- Z, the rows of T above the block and the columns of T
+// right of it -- is updated every G sweeps by a wavefront pass: rotation k of
+// logged sweep a acts at time tau = k + 2a, so one pass over a row (or
+// column) applies all G sweeps in the order of the unwindowed algorithm while
+// holding only 2G consecutive entries, in a circular register window whose
+// slot indices are static inside an unrolled block of 2G time steps.  Each
+// element outside the active block is read and written once per G sweeps.
+// Range of logged sweeps: the time window [tmin, tmax] and the columns (rows)
+// lo..hi that any rotation touches, plus the largest il and smallest iu.
+struct FlushRange {
+  int tmin, tmax, lo, hi, ilmax, iumin;
+};
+
+template <int G>
+__device__ FlushRange flush_range(int m, const int* s_il, const int* s_iu) {
+  int tmin = 1 << 30, tmax = -1, cmin = 1 << 30, cmax = -1, ilmax = -1, iumin = 1 << 30;
+  for (int a = 0; a < m; ++a) {
+    tmin = min(tmin, s_il[a] + 2 * a);
+    tmax = max(tmax, s_iu[a] - 1 + 2 * a);
+    cmin = min(cmin, s_il[a]);
+    cmax = max(cmax, s_iu[a]);
+    ilmax = max(ilmax, s_il[a]);
+    iumin = min(iumin, s_iu[a]);
+  }
+  FlushRange f;
+  f.tmin = tmin;
+  f.tmax = tmax;
+  f.lo = max(cmin, tmin - 2 * (G - 1));
+  f.hi = min(cmax, tmax + 1);
+  f.ilmax = ilmax;
+  f.iumin = iumin;
+  return f;
+}
+
+// Applies the m logged sweeps to one strip: entries base[c * step], c = lo..hi.
+// kind 0: a row of Z (every sweep), kind 1: row idx of T (sweeps with
+// idx < il), both by column rotations; kind 2: column idx of T (sweeps with
+// idx > iu) by row rotations.
+template <int G>
+__device__ void flush_strip(const FlushRange& f, int kind, int idx, zc* base, long long step,
+                            int n, int m, const int* s_il, const int* s_iu, const double* lc,
+                            const zc* ls) {
+  constexpr int W = 2 * G;  // power of two: slot of column c is c & (W - 1)
+  const int lo = f.lo, hi = f.hi, tmin = f.tmin, tmax = f.tmax;
+  // per-sweep bounds in scalar registers, the applicability test as a mask
+  int ila[G], iua[G];
+  unsigned mask = 0;
+#pragma unroll
+  for (int a = 0; a < G; ++a) {
+    ila[a] = a < m ? __builtin_amdgcn_readfirstlane(s_il[a]) : 0;
+    iua[a] = a < m ? __builtin_amdgcn_readfirstlane(s_iu[a]) : 0;
+    const bool applies = kind == 0 || (kind == 1 ? idx < ila[a] : idx > iua[a]);
+    if (a < m && applies) mask |= 1u << a;
+  }
+  zc w[W];
+  for (int c = lo; c <= min(tmin, hi); ++c) {
+    const zc v = base[(long long)c * step];
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+      if ((c & (W - 1)) == q) w[q] = v;
+  }
+  for (int tb = (tmin / W) * W; tb <= tmax; tb += W) {
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const int tau = tb + u;
+      if (tau < tmin || tau > tmax) continue;
+      const int cn = tau + 1;
+      if (cn > tmin && cn <= hi) w[(u + 1) & (W - 1)] = base[(long long)cn * step];
+#pragma unroll
+      for (int a = 0; a < G; ++a) {
+        const int k = tau - 2 * a;
+        if (!((mask >> a) & 1u) || k < ila[a] || k >= iua[a]) continue;
+        const double c = lc[(size_t)a * n + (k - ila[a])];
+        const zc sr = ls[(size_t)a * n + (k - ila[a])];
+        zc& x = w[(u - 2 * a) & (W - 1)];
+        zc& y = w[(u - 2 * a + 1) & (W - 1)];
+        const zc xv = x, yv = y;
+        if (kind == 2) {  // row rotation on a column of T
+          x = zadd(zscale(xv, c), zmul(sr, yv));
+          y = zsub(zscale(yv, c), zmul(zconj(sr), xv));
+        } else {          // column rotation on a row of T or Z
+          x = zadd(zscale(xv, c), zmul(zconj(sr), yv));
+          y = zsub(zscale(yv, c), zmul(sr, xv));
+        }
+      }
+      const int done = tau - 2 * (G - 1);  // no rotation reaches this column any more
+      if (done >= lo && done <= hi) base[(long long)done * step] = w[(u - 2 * (G - 1)) & (W - 1)];
+    }
+  }
+  for (int c = max(lo, tmax - 2 * (G - 1) + 1); c <= hi; ++c) {
+    zc v = w[0];
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+      if ((c & (W - 1)) == q) v = w[q];
+    base[(long long)c * step] = v;
+  }
+}
+
+// Rows of Z and T first, then columns of T.  A row strip and a column strip
+// of T can share entries; left and right rotations commute, but each strip
+// needs its whole row or column, so the two phases are separated by a barrier.
+// Called by all NT threads of the workgroup.
+template <int G>
+__device__ void defer_flush(int n, int m, const int* s_il, const int* s_iu, const double* lc,
+                            const zc* ls, zc* T, long long ld, zc* Z, long long ldz, int tid,
+                            int nt) {
+  const FlushRange f = flush_range<G>(m, s_il, s_iu);
+  const int nrow = n + max(f.ilmax, 0), ncol = max(n - 1 - f.iumin, 0);
+#pragma unroll 1
+  for (int ph = 0; ph < 2; ++ph) {
+    const int cnt = ph == 0 ? nrow : ncol;
+    for (int st = tid; st < cnt; st += nt) {
+      int kind, idx;
+      zc* base;
+      long long step;
+      if (ph == 1) {
+        kind = 2;
+        idx = f.iumin + 1 + st;
+        base = T + (long long)idx * ld;
+        step = 1;
+      } else if (st < n) {
+        kind = 0;
+        idx = st;
+        base = Z + st;
+        step = ldz;
+      } else {
+        kind = 1;
+        idx = st - n;
+        base = T + idx;
+        step = ld;
+      }
+      flush_strip<G>(f, kind, idx, base, step, n, m, s_il, s_iu, lc, ls);
+    }
+    __syncthreads();
+  }
+}
+
+template <int R, int NT, int G>
+__global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm, int* info,
+                                                         int maxit_per_row, double* logc,
+                                                         zc* logs) {
+  constexpr int LDB = R + 3;
+  __shared__ zc band[LDB * LDB];
+  __shared__ double rc[R];
+  __shared__ zc rs[R];
+  __shared__ int s_il, s_lil[G], s_liu[G];
+  __shared__ zc s_mu;
+  const int b = blockIdx.x;
+  zc* T = Tm.p + (long long)b * Tm.stride;
+  zc* Z = Zm.p + (long long)b * Zm.stride;
+  double* lc = logc + (size_t)b * G * n;
+  zc* ls = logs + (size_t)b * G * n;
+  const long long ld = Tm.ld, ldz = Zm.ld;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double eps = DBL_EPSILON;
+  const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
+  int iu = n - 1, iter = 0, nlog = 0;
+  long long total = 0;
+  while (iu > 0) {
+    if (tid == 0) s_il = 0;
+    __syncthreads();
+    for (int k = iu - tid; k >= 1; k -= NT) {
+      const double sub = zabs1(T[k + (k - 1) * ld]);
+      const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
+      if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
+    }
+    __syncthreads();
+    const int il = s_il;
+    if (il == iu) {
+      if (tid == 0) T[iu + (iu - 1) * ld] = zmake(0, 0);
+      --iu;
+      iter = 0;
+      __syncthreads();
+      continue;
+    }
+    ++iter;
+    ++total;
+    if (total > maxit) {
+      if (tid == 0 && info) info[b] = 1;
+      break;
+    }
+    if (tid == 0) {
+      if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
+      zc mu;
+      if (iter == 10)
+        mu = zadd(T[il + il * ld], zmake(0.75 * fabs(T[(il + 1) + il * ld].x), 0));
+      else if (iter == 20)
+        mu = zadd(T[iu + iu * ld], zmake(0.75 * fabs(T[iu + (iu - 1) * ld].x), 0));
+      else
+        mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
+                       T[iu + (iu - 1) * ld], T[iu + iu * ld]);
+      s_mu = mu;
+      s_lil[nlog] = il;
+      s_liu[nlog] = iu;
+    }
+    __syncthreads();
+    for (int k0 = il; k0 < iu;) {
+      const int k1 = min(k0 + R, iu);
+      const int nr = k1 - k0;
+      const int b0 = k0 > 0 ? k0 - 1 : 0;
+      const int be = min(k1 + 1, n - 1);
+      const int nb = be - b0 + 1;
+      for (int idx = tid; idx < nb * nb; idx += NT) {
+        const int i = idx % nb, j = idx / nb;
+        band[i + j * LDB] = T[(b0 + i) + (long long)(b0 + j) * ld];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const int je = min(be, iu);
+        for (int k = k0; k < k1; ++k) {
+          double c = 0.0;
+          zc sr = zmake(0, 0);
+          if (lane == 0) {
+            zc f, g, r;
+            if (k == il) {
+              f = zsub(band[(k - b0) + (k - b0) * LDB], s_mu);
+              g = band[(k + 1 - b0) + (k - b0) * LDB];
+            } else {
+              f = band[(k - b0) + (k - 1 - b0) * LDB];
+              g = band[(k + 1 - b0) + (k - 1 - b0) * LDB];
+            }
+            givens(f, g, c, sr, r);
+            if (k > il) {
+              band[(k - b0) + (k - 1 - b0) * LDB] = r;
+              band[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
+            }
+            rc[k - k0] = c;
+            rs[k - k0] = sr;
+          }
+          c = __shfl(c, 0);
+          sr = zmake(__shfl(sr.x, 0), __shfl(sr.y, 0));
+          const zc sc = zconj(sr);
+          for (int j = k + lane; j <= je; j += 64) {
+            zc* p0 = &band[(k - b0) + (j - b0) * LDB];
+            const zc x = p0[0], y = p0[1];
+            p0[0] = zadd(zscale(x, c), zmul(sr, y));
+            p0[1] = zsub(zscale(y, c), zmul(sc, x));
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          const int ie = min(k + 2, iu);
+          for (int i = max(b0, il) + lane; i <= ie; i += 64) {
+            zc* q0 = &band[(i - b0) + (k - b0) * LDB];
+            const zc x = q0[0], y = q0[LDB];
+            q0[0] = zadd(zscale(x, c), zmul(sc, y));
+            q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __syncthreads();
+      for (int idx = tid; idx < nb * nb; idx += NT) {
+        const int i = idx % nb, j = idx / nb;
+        T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
+      }
+      if (tid < nr) {
+        lc[(size_t)nlog * n + (k0 - il) + tid] = rc[tid];
+        ls[(size_t)nlog * n + (k0 - il) + tid] = rs[tid];
+      }
+      // rows k0..k1 of the active block right of the band
+      for (int j = be + 1 + tid; j <= iu; j += NT) {
+        zc v[R + 1];
+        zc* col = T + k0 + (long long)j * ld;
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) v[q] = col[q];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (q < nr) {
+            const double c = rc[q];
+            const zc sr = rs[q], sc = zconj(sr);
+            const zc x = v[q], y = v[q + 1];
+            v[q] = zadd(zscale(x, c), zmul(sr, y));
+            v[q + 1] = zsub(zscale(y, c), zmul(sc, x));
+          }
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) col[q] = v[q];
+      }
+      // columns k0..k1 of the active block above the band
+      for (int i = il + tid; i < b0; i += NT) {
+        zc* base = T + i + (long long)k0 * ld;
+        zc v[R + 1];
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) v[q] = base[q * ld];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+          if (q < nr) {
+            const double c = rc[q];
+            const zc sr = rs[q], sc = zconj(sr);
+            const zc x = v[q], y = v[q + 1];
+            v[q] = zadd(zscale(x, c), zmul(sc, y));
+            v[q + 1] = zsub(zscale(y, c), zmul(sr, x));
+          }
+#pragma unroll
+        for (int q = 0; q <= R; ++q)
+          if (q <= nr) base[q * ld] = v[q];
+      }
+      __syncthreads();
+      k0 = k1;
+    }
+    ++nlog;
+    if (nlog == G) {
+      __threadfence_block();
+      defer_flush<G>(n, nlog, s_lil, s_liu, lc, ls, T, ld, Z, ldz, tid, NT);
+      __syncthreads();
+      nlog = 0;
+    }
+  }
+  __syncthreads();
+  if (nlog > 0) defer_flush<G>(n, nlog, s_lil, s_liu, lc, ls, T, ld, Z, ldz, tid, NT);
+}
+
 __global__ __launch_bounds__(64) void trevc_kernel(int n, ZCMat Tm, ZMat Xm) {
   extern __shared__ zc rhs[];
   const int k = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
@@ -457,15 +777,27 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch) {
 int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                              int maxit_per_row);
 
+size_t schur_ws_bytes(int n, int batch) {
+  return (size_t)batch * SCHUR_G * n * (sizeof(double) + sizeof(zc)) + 512;
+}
+
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
-                  int maxit_per_row) {
+                  int maxit_per_row, void* ws) {
   if (n <= 0 || batch <= 0) return 0;
-  // RCWA_QR_SWEEP=direct selects the unwindowed sweep (same arithmetic).
-  static const bool direct = [] {
+  // RCWA_QR_SWEEP=direct selects the unwindowed sweep, =window the windowed
+  // sweep that updates all of T and Z at once (same arithmetic, same result).
+  static const char mode = [] {
     const char* e = getenv("RCWA_QR_SWEEP");
-    return e && e[0] == 'd';
+    return e ? e[0] : 0;
   }();
-  if (direct) return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
+  if (mode == 'd') return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
+  if (ws && mode != 'w') {
+    double* lc = (double*)ws;
+    zc* ls = (zc*)(((uintptr_t)(lc + (size_t)batch * SCHUR_G * n) + 255) & ~(uintptr_t)255);
+    hipLaunchKernelGGL((schur_defer_kernel<12, 256, SCHUR_G>), dim3(batch), dim3(256), 0, s, n,
+                       T, Z, info, maxit_per_row, lc, ls);
+    return (int)hipGetLastError();
+  }
   hipLaunchKernelGGL((schur_win_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z, info,
                      maxit_per_row);
   return (int)hipGetLastError();
@@ -486,7 +818,8 @@ int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch) {
 }
 
 size_t eig_workspace_bytes(int n, int batch) {
-  return 2 * (size_t)batch * n * n * sizeof(zc) + hess_blocked_ws_bytes(n, batch) + 512;
+  return 2 * (size_t)batch * n * n * sizeof(zc) + hess_blocked_ws_bytes(n, batch) +
+         schur_ws_bytes(n, batch) + 1024;
 }
 
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
@@ -495,6 +828,7 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   zc* Zp = (zc*)ws;
   zc* Xp = Zp + (size_t)batch * n * n;
   ZMat Z{Zp, n, (long long)n * n}, X{Xp, n, (long long)n * n};
+  void* hws = (void*)(((uintptr_t)(Xp + (size_t)batch * n * n) + 255) & ~(uintptr_t)255);
   int e;
   if (marks) (void)hipEventRecord(marks[0], s);
   // RCWA_HESS=unblocked selects the one-workgroup unblocked reduction.
@@ -505,11 +839,11 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   if (unblocked || n > 4096) {
     if ((e = hessenberg_batched(s, n, A, Z, batch))) return e;
   } else {
-    void* hws = (void*)(((uintptr_t)(Xp + (size_t)batch * n * n) + 255) & ~(uintptr_t)255);
     if ((e = hessenberg_blocked_batched(s, n, A, Z, batch, hws))) return e;
   }
   if (marks) (void)hipEventRecord(marks[1], s);
-  if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row))) return e;
+  void* sws = (void*)(((uintptr_t)hws + hess_blocked_ws_bytes(n, batch) + 255) & ~(uintptr_t)255);
+  if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws))) return e;
   if (marks) (void)hipEventRecord(marks[2], s);
   if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch))) return e;
   hipLaunchKernelGGL(diag_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n,

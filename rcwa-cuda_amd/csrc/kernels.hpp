@@ -45,7 +45,11 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
 // Blocked (compact-WY panels + ZGEMM updates); ws holds hess_blocked_ws_bytes(n, batch).
 size_t hess_blocked_ws_bytes(int n, int batch);
 int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, void* ws);
-int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row);
+// ws (schur_ws_bytes) enables the deferred-update kernel; null selects the
+// windowed kernel that updates all of T and Z per window.
+size_t schur_ws_bytes(int n, int batch);
+int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row,
+                  void* ws = nullptr);
 // The same sweeps with every rotation applied directly in global memory.
 int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                              int maxit_per_row);

@@ -111,6 +111,12 @@ void* Ctx::take(size_t bytes) {
   return p;
 }
 
+rcwa_status Ctx::arena_check() const {
+  if (arena_off > arena_cap)
+    return rcwa_set_error(RCWA_E_NOMEM, "internal: arena plan exceeded its reservation");
+  return RCWA_OK;
+}
+
 // ------------------------------------------------------------------ GEMM dispatch
 int Ctx::gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C,
               int batch) {
@@ -133,8 +139,9 @@ int Ctx::gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D,
 // ------------------------------------------------------------------ layer stage
 size_t layer_work_bytes(int N, int Bl) {
   const size_t n = 2 * (size_t)N, n2 = n * n;
-  size_t z = (size_t)Bl * (8 * n2 + 2 * (size_t)N * N);
-  return z * sizeof(zc) + gj_workspace_bytes((int)n, Bl) + eig_workspace_bytes((int)n, 0) + 4096;
+  const size_t z = (size_t)Bl * (6 * n2 + 2 * (size_t)N * N);
+  return z * sizeof(zc) + gj_workspace_bytes((int)n, Bl) + eig_workspace_bytes((int)n, Bl) +
+         10 * 256 + 4096;
 }
 
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
@@ -395,8 +402,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   void* gjf = c.take(gj_workspace_bytes(n, std::max(G, 1)));
   LayerWork w;
   w.carve(c, N, Bl);
-  if (c.arena_off > c.arena_cap)
-    return rcwa_set_error(RCWA_E_NOMEM, "internal: arena plan exceeded its reservation");
+  CK(c.arena_check());
 
   // layer-major destination of job g's layer l
   std::vector<int> jmap((size_t)std::max(Lmax, 1) * G, 0);

@@ -78,6 +78,7 @@ rcwa_status rcwa_convmat(rcwa_ctx* ctx, const rcwa_c128* field, int layer, int N
   zc* rb = (zc*)c.take((size_t)Ny * (4 * hx + 1) * sizeof(zc));
   zc* F = (zc*)c.take(nf * sizeof(zc));
   zc* Cm = (zc*)c.take((size_t)PQ * PQ * sizeof(zc));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   if ((st = h2d(c, g, field + (size_t)layer * G, G * sizeof(zc)))) return st;
   CK(convmat_dft(c.stream, g, Nx, Ny, hx, hy, F, 1, rb));
   CK(convmat_gather(c.stream, F, Nx, Ny, hx, hy, ZMat{Cm, PQ, (long long)PQ * PQ}, 1));
@@ -97,6 +98,7 @@ rcwa_status rcwa_convmat_batched(rcwa_ctx* ctx, const rcwa_c128* grids_dev, int 
   if (st) return st;
   zc* rb = (zc*)c.take((size_t)batch * Ny * (4 * hx + 1) * sizeof(zc));
   zc* F = (zc*)c.take((size_t)batch * nf * sizeof(zc));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   CK(convmat_dft(c.stream, (const zc*)grids_dev, Nx, Ny, hx, hy, F, batch, rb));
   CK(convmat_gather(c.stream, F, Nx, Ny, hx, hy, ZMat{(zc*)C_dev, PQ, (long long)PQ * PQ}, batch));
   CKH(hipStreamSynchronize(c.stream));
@@ -174,7 +176,7 @@ rcwa_status rcwa_layer_smatrix(rcwa_ctx* ctx, int PQ, int batch, const rcwa_c128
   const size_t n2 = (size_t)n * n, N2 = (size_t)N * N;
   const size_t bytes = (2 * batch * N2 + 2 * batch * n2 + batch * (6 * N + 4 * n)) * sizeof(zc) +
                        batch * (sizeof(double) + 2 * sizeof(int)) + layer_work_bytes(N, batch) +
-                       eig_workspace_bytes(n, batch) + 64 * 256;
+                       64 * 256;
   rcwa_status st = c.reserve(bytes);
   if (st) return st;
   zc* derc = (zc*)c.take(batch * N2 * sizeof(zc));
@@ -192,6 +194,7 @@ rcwa_status rcwa_layer_smatrix(rcwa_ctx* ctx, int PQ, int batch, const rcwa_c128
   int* dinfo = (int*)c.take(2 * batch * sizeof(int));
   LayerWork w;
   w.carve(c, N, batch);
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   std::vector<cplx> kx = vec(Kx, N), ky = vec(Ky, N);
   std::vector<cplx> v0 = rcwa_host::gap_v0_diagonals(kx, ky);
   std::vector<cplx> hKx((size_t)batch * N), hKy((size_t)batch * N), hv0((size_t)batch * 4 * N);
@@ -239,6 +242,7 @@ rcwa_status rcwa_star_product(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* 
   for (auto& p : t) p = (zc*)c.take(blk);
   void* gj = c.take(gj_workspace_bytes(n, batch));
   int* dinfo = (int*)c.take(batch * sizeof(int));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   for (int k = 0; k < 4; ++k) {
     if (!A[k] || !B[k] || !S[k]) return rcwa_set_error(RCWA_E_ARG, "null block");
     if ((st = h2d(c, a[k], A[k], blk))) return st;
@@ -269,6 +273,7 @@ rcwa_status rcwa_eig(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* A, rcwa_c
   zc* dw = (zc*)c.take(batch * n * sizeof(zc));
   void* ws = c.take(eig_workspace_bytes(n, batch));
   int* dinfo = (int*)c.take(batch * sizeof(int));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   if ((st = h2d(c, dA, A, blk))) return st;
   CKH(hipMemsetAsync(dinfo, 0, batch * sizeof(int), c.stream));
   CK(eig_batched(c.stream, n, ZMat{dA, n, (long long)n2}, dw, ZMat{dV, n, (long long)n2}, batch,
@@ -291,6 +296,7 @@ rcwa_status rcwa_inverse(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* A, rc
   zc* dA = (zc*)c.take(blk);
   void* ws = c.take(gj_workspace_bytes(n, batch));
   int* dinfo = (int*)c.take(batch * sizeof(int));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   if ((st = h2d(c, dA, A, blk))) return st;
   CKH(hipMemsetAsync(dinfo, 0, batch * sizeof(int), c.stream));
   CK(gj_inverse_batched(c.stream, n, ZMat{dA, n, (long long)n2}, batch, ws, dinfo));
@@ -310,6 +316,7 @@ static rcwa_status zgemm_impl(rcwa_ctx* ctx, int m, int n, int k, int batch, con
   zc* dA = (zc*)c.take(batch * sa * sizeof(zc));
   zc* dB = (zc*)c.take(batch * sb * sizeof(zc));
   zc* dC = (zc*)c.take(batch * sc * sizeof(zc));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
   if ((st = h2d(c, dA, A, batch * sa * sizeof(zc)))) return st;
   if ((st = h2d(c, dB, B, batch * sb * sizeof(zc)))) return st;
   c.precision = prec;

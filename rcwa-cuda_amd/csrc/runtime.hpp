@@ -37,6 +37,9 @@ struct Ctx {
   void release();
   rcwa_status reserve(size_t bytes);  // arena of at least `bytes`; resets the bump pointer
   void* take(size_t bytes);
+  // Error if the buffers taken since reserve() overran the arena; called after
+  // carving, before any kernel sees the buffers.
+  rcwa_status arena_check() const;
   int gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch);
 };
 

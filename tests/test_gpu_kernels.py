@@ -67,6 +67,38 @@ def test_eig_residual_and_values(ctx, n):
         assert _match_eigs(wi, np.linalg.eigvals(a)) < 1e-10 * nrm
 
 
+@pytest.mark.parametrize("n,cuts", [(60, [20]), (242, [80, 160]), (300, [7, 150, 290])])
+def test_eig_block_triangular(ctx, n, cuts):
+    """Exact zero blocks below the diagonal survive the Hessenberg reduction,
+    so the QR iteration splits in the middle of the matrix: the active block
+    [il, iu] moves up and down, and updates deferred for rows above il and
+    columns right of iu must still land."""
+    rng = np.random.default_rng(n)
+    a = rnd(rng, n, n)
+    edges = [0] + cuts + [n]
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        a[hi:, lo:hi] = 0
+    w, V = ctx.eig([a, a.T.copy()])
+    for m, wi, vi in zip([a, a.T], w, V):
+        nrm = np.abs(m).max()
+        res = np.abs(m @ vi - vi * wi[None, :]).max() / nrm
+        assert res < 1e-12 * n, res
+        assert _match_eigs(wi, np.linalg.eigvals(m)) < 1e-10 * nrm
+
+
+def test_eig_workload_size(ctx):
+    """n = 882: the Omega^2 size of 21 x 21 harmonics."""
+    n = 882
+    rng = np.random.default_rng(882)
+    A = [rnd(rng, n, n) for _ in range(2)]
+    w, V = ctx.eig(A)
+    for a, wi, vi in zip(A, w, V):
+        nrm = np.abs(a).max()
+        res = np.abs(a @ vi - vi * wi[None, :]).max() / nrm
+        assert res < 1e-12 * n, res
+        assert _match_eigs(wi, np.linalg.eigvals(a)) < 1e-9 * nrm
+
+
 def test_eig_degenerate_diagonal(ctx):
     # a uniform layer gives a diagonal Omega^2 with every eigenvalue twice
     d = np.repeat(np.arange(1, 6) - 3.5 + 0.1j, 2)
