@@ -211,7 +211,8 @@ def main():
     per_launch_mats = st["n_eig_matrices"] / launches
     avg_ms = st["ms_eig_schur"] / launches
     achieved = f_qr * per_launch_mats / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    roofline = {"kernel": "schur_defer_kernel (complex single-shift QR sweeps, T and Z)",
+    roofline = {"kernel": "QR phase: schur_defer_kernel + 2 x qr_apply_kernel (complex "
+                          "single-shift QR sweeps, Schur form and Schur vectors)",
                 "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                 "avg_launch_ms": avg_ms, "matrices_per_launch": per_launch_mats}

@@ -16,6 +16,7 @@
 // The S-matrices the path builds from W depend only on the spans of
 // eigenvectors and not on their scale or order, so W need not match Eigen's
 // normalisation or sort order.
+#include <algorithm>
 #include <cfloat>
 #include <cstdlib>
 #include "common.hpp"
@@ -25,7 +26,27 @@ namespace {
 
 constexpr int HESS_THREADS = 1024;
 constexpr int QR_THREADS = 512;
-constexpr int SCHUR_G = 8;  // sweeps per deferred-update pass
+// Rotation logs of a batch of QR iterations (per matrix: room for cap
+// rotations and scap sweeps, of which at most limit rotations are used before
+// the sweep kernel applies the log itself; nsw[b] = sweeps logged and not yet
+// applied).
+struct QrLog {
+  double* c;
+  zc* s;
+  int* il;
+  int* iu;
+  int* off;
+  int* nsw;
+  long long cap;
+  long long limit;
+  int scap;
+};
+
+constexpr int SCHUR_G = 4;     // sweeps per pass of the in-kernel fallback
+constexpr int FLUSH_TC = 128;  // time steps per LDS-staged chunk of that pass
+constexpr int APPLY_G = 8;     // sweeps per pass of qr_apply_kernel
+constexpr int APPLY_TC = 128;  // time steps per LDS-staged chunk of qr_apply_kernel
+constexpr int APPLY_NT = 256;
 
 __device__ __forceinline__ zc shfl_down_z(zc v, int off) {
   return zmake(__shfl_down(v.x, off), __shfl_down(v.y, off));
@@ -413,98 +434,225 @@ __global__ __launch_bounds__(NT) void schur_win_kernel(int n, ZMat Tm, ZMat Zm, 
 
 // ---------------------------------------------------------------------------
 // QR sweeps with deferred updates.  The iteration itself only ever reads the
-// active block T[il:iu+1, il:iu+1], so each sweep updates just that block
-// (windowed as in schur_win_kernel) and logs its rotations.  Everything else
-// the sweep changes -- Z, the rows of T above the block and the columns of T
-// right of it -- is updated every G sweeps by a wavefront pass: rotation k of
-// logged sweep a acts at time tau = k + 2a, so one pass over a row (or
-// column) applies all G sweeps in the order of the unwindowed algorithm while
-// holding only 2G consecutive entries, in a circular register window whose
-// slot indices are static inside an unrolled block of 2G time steps.  Each
-// element outside the active block is read and written once per G sweeps.
-// Range of logged sweeps: the time window [tmin, tmax] and the columns (rows)
-// lo..hi that any rotation touches, plus the largest il and smallest iu.
-struct FlushRange {
+// active block T[il:iu+1, il:iu+1] (the block only shrinks, or moves to rows
+// and columns that no sweep since its last visit has touched), so each sweep
+// updates just that block (windowed as in schur_win_kernel) and logs its
+// rotations.  Everything else the sweeps change -- Z, the rows of T above the
+// block and the columns of T right of it -- is applied after the iteration by
+// qr_apply_kernel, which spreads the rows and columns of every matrix over the
+// whole chip (or in the sweep kernel itself when its log fills up).  A pass
+// applies G logged sweeps at once in wavefront order: rotation k of sweep a
+// acts at time tau = k + 2a, so one pass over a row (or column) applies the G
+// sweeps in the order of the unwindowed algorithm while holding only 2G
+// consecutive entries, in a circular register window whose slot indices are
+// static inside an unrolled block of 2G time steps.  Each element outside the
+// active block is read and written once per G sweeps.
+// The rotation log of one matrix: sweep q acted on rows/columns il[q]..iu[q]
+// with rotations (c, s)[off[q] + k - il[q]], k = il[q]..iu[q]-1.
+struct QrLogM {
+  double* c;
+  zc* s;
+  int* il;
+  int* iu;
+  int* off;
+};
+
+__device__ __forceinline__ QrLogM qr_log_of(const QrLog& L, int b) {
+  QrLogM m;
+  m.c = L.c + (size_t)b * L.cap;
+  m.s = L.s + (size_t)b * L.cap;
+  m.il = L.il + (size_t)b * L.scap;
+  m.iu = L.iu + (size_t)b * L.scap;
+  m.off = L.off + (size_t)b * L.scap;
+  return m;
+}
+
+// Read-only views of one matrix's rotation log in the constant address
+// space, for qr_apply_kernel, a launch in which the log is never written:
+// uniform indices make these scalar loads into SGPRs.  (The sweep kernel,
+// which writes the log, reads it through QrLogM.)
+typedef const __attribute__((address_space(4))) double* KDouble;
+typedef const __attribute__((address_space(4))) int* KInt;
+
+struct QrLogK {
+  KDouble c;
+  KDouble s;  // (re, im) pairs
+  KInt il;
+  KInt iu;
+  KInt off;
+};
+
+__device__ __forceinline__ QrLogK qr_log_k(const QrLog& L, int b) {
+  QrLogK k;
+  k.c = (KDouble)(L.c + (size_t)b * L.cap);
+  k.s = (KDouble)(L.s + (size_t)b * L.cap);
+  k.il = (KInt)(L.il + (size_t)b * L.scap);
+  k.iu = (KInt)(L.iu + (size_t)b * L.scap);
+  k.off = (KInt)(L.off + (size_t)b * L.scap);
+  return k;
+}
+
+// One group of m <= G consecutive logged sweeps (first sweep g0).
+template <int G>
+struct SweepGroup {
+  int m, il[G], iu[G], off[G];
   int tmin, tmax, lo, hi, ilmax, iumin;
 };
 
-template <int G>
-__device__ FlushRange flush_range(int m, const int* s_il, const int* s_iu) {
+// LogView: QrLogK (scalar loads; only where the log is read-only for the
+// whole kernel) or QrLogM (ordinary loads; in the kernel that writes the log).
+template <int G, typename LogView>
+__device__ __forceinline__ SweepGroup<G> load_group(const LogView& L, int g0, int m) {
+  SweepGroup<G> g;
+  g.m = m;
   int tmin = 1 << 30, tmax = -1, cmin = 1 << 30, cmax = -1, ilmax = -1, iumin = 1 << 30;
-  for (int a = 0; a < m; ++a) {
-    tmin = min(tmin, s_il[a] + 2 * a);
-    tmax = max(tmax, s_iu[a] - 1 + 2 * a);
-    cmin = min(cmin, s_il[a]);
-    cmax = max(cmax, s_iu[a]);
-    ilmax = max(ilmax, s_il[a]);
-    iumin = min(iumin, s_iu[a]);
-  }
-  FlushRange f;
-  f.tmin = tmin;
-  f.tmax = tmax;
-  f.lo = max(cmin, tmin - 2 * (G - 1));
-  f.hi = min(cmax, tmax + 1);
-  f.ilmax = ilmax;
-  f.iumin = iumin;
-  return f;
-}
-
-// Applies the m logged sweeps to one strip: entries base[c * step], c = lo..hi.
-// kind 0: a row of Z (every sweep), kind 1: row idx of T (sweeps with
-// idx < il), both by column rotations; kind 2: column idx of T (sweeps with
-// idx > iu) by row rotations.
-template <int G>
-__device__ void flush_strip(const FlushRange& f, int kind, int idx, zc* base, long long step,
-                            int n, int m, const int* s_il, const int* s_iu, const double* lc,
-                            const zc* ls) {
-  constexpr int W = 2 * G;  // power of two: slot of column c is c & (W - 1)
-  const int lo = f.lo, hi = f.hi, tmin = f.tmin, tmax = f.tmax;
-  // per-sweep bounds in scalar registers, the applicability test as a mask
-  int ila[G], iua[G];
-  unsigned mask = 0;
 #pragma unroll
   for (int a = 0; a < G; ++a) {
-    ila[a] = a < m ? __builtin_amdgcn_readfirstlane(s_il[a]) : 0;
-    iua[a] = a < m ? __builtin_amdgcn_readfirstlane(s_iu[a]) : 0;
-    const bool applies = kind == 0 || (kind == 1 ? idx < ila[a] : idx > iua[a]);
-    if (a < m && applies) mask |= 1u << a;
+    // sweeps a >= m get an empty range, so they never rotate anything
+    g.il[a] = a < m ? L.il[g0 + a] : 0;
+    g.iu[a] = a < m ? L.iu[g0 + a] : 0;
+    g.off[a] = a < m ? L.off[g0 + a] : 0;
+    if (a < m) {
+      tmin = min(tmin, g.il[a] + 2 * a);
+      tmax = max(tmax, g.iu[a] - 1 + 2 * a);
+      cmin = min(cmin, g.il[a]);
+      cmax = max(cmax, g.iu[a]);
+      ilmax = max(ilmax, g.il[a]);
+      iumin = min(iumin, g.iu[a]);
+    }
   }
+  g.tmin = tmin;
+  g.tmax = tmax;
+  g.lo = max(cmin, tmin - 2 * (G - 1));  // first column (row) any rotation touches
+  g.hi = min(cmax, tmax + 1);            // last one
+  g.ilmax = ilmax;
+  g.iumin = iumin;
+  return g;
+}
+
+// Strip st of a deferred pass.  Phase 0: st < n is row st of Z, st >= n row
+// st - n of T (both take column rotations); phase 1: column st of T (row
+// rotations).  Returns the sweeps of the group that apply to it as a mask.
+template <int G>
+__device__ __forceinline__ unsigned strip_of(int ph, int st, int n, const SweepGroup<G>& g,
+                                             zc* T, long long ld, zc* Z, long long ldz,
+                                             zc*& base, long long& step) {
+  unsigned mask = 0;
+  if (ph == 1) {
+    base = T + (long long)st * ld;
+    step = 1;
+#pragma unroll
+    for (int a = 0; a < G; ++a) mask |= (a < g.m && st > g.iu[a] ? 1u : 0u) << a;
+  } else if (st < n) {
+    base = Z + st;
+    step = ldz;
+    mask = (1u << g.m) - 1u;
+  } else {
+    base = T + (st - n);
+    step = ld;
+#pragma unroll
+    for (int a = 0; a < G; ++a) mask |= (a < g.m && st - n < g.il[a] ? 1u : 0u) << a;
+  }
+  return mask;
+}
+
+// LDS table of one chunk of time steps T0 .. T0+TC-1: entry a * TC + u holds
+// the rotation of sweep a of the group at time T0 + u, the identity where that
+// sweep has none, conjugated for row rotations (ph == 1).
+template <int G, int TC>
+__device__ void fill_table(const SweepGroup<G>& g, const QrLogM& L, int T0, int ph, double* tc,
+                           zc* ts, int tid, int nt) {
+  for (int e = tid; e < G * TC; e += nt) {
+    const int a = e / TC, k = T0 + (e % TC) - 2 * a;
+    double c = 1.0;
+    zc sr = zmake(0, 0);
+    int il = 0, iu = 0, off = 0;
+#pragma unroll
+    for (int q = 0; q < G; ++q)
+      if (q == a) {
+        il = g.il[q];
+        iu = g.iu[q];
+        off = g.off[q];
+      }
+    if (k >= il && k < iu) {
+      c = L.c[off + (k - il)];
+      sr = L.s[off + (k - il)];
+      if (ph == 1) sr = zconj(sr);
+    }
+    tc[e] = c;
+    ts[e] = sr;
+  }
+}
+
+// Applies a group of logged sweeps to one strip, entries base[c * step],
+// c = lo..hi, for the chunk T0 .. T1 of the group's time range, as column
+// rotations read from the LDS table (fill_table conjugates them for strips
+// that take row rotations).  Sweeps outside `mask` act as the identity
+// (MASKED: some do).  Rotation k of sweep a acts at time tau = k + 2a, so one
+// pass applies the group in the order of the unwindowed algorithm while
+// holding only the entries in flight, in a circular register window (slot of
+// column c: c mod W) whose slot indices are static inside an unrolled block of
+// W time steps.  Every step of a block runs unconditionally -- a time with no
+// rotation for a sweep meets the identity -- so that the window stays in fixed
+// registers.  Entries still in flight at the chunk end are stored back and
+// reloaded by the next chunk, so chunks compose exactly.
+template <int G, int TC, bool MASKED>
+__device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsigned mask, zc* base,
+                                  long long step, const double* tc, const zc* ts) {
+  constexpr int W = 2 * G;
+  const int lo = g.lo, hi = g.hi, first = g.tmin - 2 * (G - 1);
+  // Slots that hold no column of the strip stay zero: identity rotations
+  // still combine them, and 1 * x + 0 * y must not meet garbage.
   zc w[W];
-  for (int c = lo; c <= min(tmin, hi); ++c) {
+#pragma unroll
+  for (int q = 0; q < W; ++q) w[q] = zmake(0, 0);
+  for (int c = max(lo, max(T0, g.tmin) - 2 * (G - 1)); c <= min(T0, hi); ++c) {
     const zc v = base[(long long)c * step];
 #pragma unroll
     for (int q = 0; q < W; ++q)
       if ((c & (W - 1)) == q) w[q] = v;
   }
-  for (int tb = (tmin / W) * W; tb <= tmax; tb += W) {
+  int tb = T0;
+  for (; tb <= T1; tb += W) {
+    // The block's incoming columns tb+1 .. tb+W, loaded together.  The slot of
+    // column cn is free when step cn - 1 fills it: its previous column,
+    // cn - W, was stored at step cn - 2.  A column the preload already holds
+    // is reloaded unchanged (no rotation has reached it yet).
+    zc pre[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const int cn = tb + u + 1;
+      pre[u] = zmake(0, 0);
+      if (cn >= lo && cn <= hi && cn >= first) pre[u] = base[(long long)cn * step];
+    }
 #pragma unroll
     for (int u = 0; u < W; ++u) {
       const int tau = tb + u;
-      if (tau < tmin || tau > tmax) continue;
-      const int cn = tau + 1;
-      if (cn > tmin && cn <= hi) w[(u + 1) & (W - 1)] = base[(long long)cn * step];
+      w[(u + 1) & (W - 1)] = pre[u];
 #pragma unroll
       for (int a = 0; a < G; ++a) {
-        const int k = tau - 2 * a;
-        if (!((mask >> a) & 1u) || k < ila[a] || k >= iua[a]) continue;
-        const double c = lc[(size_t)a * n + (k - ila[a])];
-        const zc sr = ls[(size_t)a * n + (k - ila[a])];
+        double c = tc[a * TC + u + (tb - T0)];
+        zc sr = ts[a * TC + u + (tb - T0)];
+        if (MASKED && !((mask >> a) & 1u)) {
+          c = 1.0;
+          sr = zmake(0, 0);
+        }
         zc& x = w[(u - 2 * a) & (W - 1)];
         zc& y = w[(u - 2 * a + 1) & (W - 1)];
         const zc xv = x, yv = y;
-        if (kind == 2) {  // row rotation on a column of T
-          x = zadd(zscale(xv, c), zmul(sr, yv));
-          y = zsub(zscale(yv, c), zmul(zconj(sr), xv));
-        } else {          // column rotation on a row of T or Z
-          x = zadd(zscale(xv, c), zmul(zconj(sr), yv));
-          y = zsub(zscale(yv, c), zmul(sr, xv));
-        }
+        x = zadd(zscale(xv, c), zmul(zconj(sr), yv));
+        y = zsub(zscale(yv, c), zmul(sr, xv));
       }
-      const int done = tau - 2 * (G - 1);  // no rotation reaches this column any more
-      if (done >= lo && done <= hi) base[(long long)done * step] = w[(u - 2 * (G - 1)) & (W - 1)];
+      // No rotation reaches column `done` any more: store it before step
+      // done + W - 1 reuses its slot (also past T1, where every rotation is
+      // the identity and the loads are zero).
+      const int done = tau - 2 * (G - 1);
+      if (done >= lo && done <= hi && done >= first)
+        base[(long long)done * step] = w[(u - 2 * (G - 1)) & (W - 1)];
     }
   }
-  for (int c = max(lo, tmax - 2 * (G - 1) + 1); c <= hi; ++c) {
+  const int tend = tb - 1;  // last executed step; later columns are still in flight
+  for (int c = max(lo, tend - 2 * (G - 1) + 1); c <= min(tend + 1, hi); ++c) {
     zc v = w[0];
 #pragma unroll
     for (int q = 0; q < W; ++q)
@@ -513,67 +661,106 @@ __device__ void flush_strip(const FlushRange& f, int kind, int idx, zc* base, lo
   }
 }
 
-// Rows of Z and T first, then columns of T.  A row strip and a column strip
-// of T can share entries; left and right rotations commute, but each strip
-// needs its whole row or column, so the two phases are separated by a barrier.
-// Called by all NT threads of the workgroup.
-template <int G>
-__device__ void defer_flush(int n, int m, const int* s_il, const int* s_iu, const double* lc,
-                            const zc* ls, zc* T, long long ld, zc* Z, long long ldz, int tid,
-                            int nt) {
-  const FlushRange f = flush_range<G>(m, s_il, s_iu);
-  const int nrow = n + max(f.ilmax, 0), ncol = max(n - 1 - f.iumin, 0);
+// Applies the first nsw logged sweeps of this workgroup's matrix to Z and to
+// T outside each sweep's active block, G sweeps per pass: rows of Z and T
+// first, then columns of T.  A row strip and a column strip of T can share
+// entries; left and right rotations commute, but each strip needs its whole
+// row or column, so the two phases are separated by a barrier.  Called by all
+// nt threads of the workgroup: the fallback of the sweep kernel when its log
+// is full.  The log was written by this kernel, so it is read with ordinary
+// loads into the LDS table (tc, ts: G * TC entries).
+template <int G, int TC>
+__device__ void flush_log(int n, int nsw, const QrLogM& L, zc* T, long long ld, zc* Z,
+                          long long ldz, int tid, int nt, double* tc, zc* ts) {
 #pragma unroll 1
   for (int ph = 0; ph < 2; ++ph) {
-    const int cnt = ph == 0 ? nrow : ncol;
-    for (int st = tid; st < cnt; st += nt) {
-      int kind, idx;
-      zc* base;
-      long long step;
-      if (ph == 1) {
-        kind = 2;
-        idx = f.iumin + 1 + st;
-        base = T + (long long)idx * ld;
-        step = 1;
-      } else if (st < n) {
-        kind = 0;
-        idx = st;
-        base = Z + st;
-        step = ldz;
-      } else {
-        kind = 1;
-        idx = st - n;
-        base = T + idx;
-        step = ld;
+    for (int g0 = 0; g0 < nsw; g0 += G) {
+      const SweepGroup<G> g = load_group<G>(L, g0, min(G, nsw - g0));
+      const int first = ph == 0 ? 0 : g.iumin + 1;
+      const int last = ph == 0 ? n + g.ilmax : n;  // exclusive
+      const int t0 = (g.tmin / (2 * G)) * (2 * G);
+      for (int T0 = t0; T0 <= g.tmax; T0 += TC) {
+        const int T1 = min(T0 + TC - 1, g.tmax);
+        __syncthreads();
+        fill_table<G, TC>(g, L, T0, ph, tc, ts, tid, nt);
+        __syncthreads();
+        for (int st = first + tid; st < last; st += nt) {
+          zc* base;
+          long long step;
+          const unsigned mask = strip_of<G>(ph, st, n, g, T, ld, Z, ldz, base, step);
+          if (mask == (1u << g.m) - 1u)
+            apply_chunk_strip<G, TC, false>(g, T0, T1, mask, base, step, tc, ts);
+          else if (mask)
+            apply_chunk_strip<G, TC, true>(g, T0, T1, mask, base, step, tc, ts);
+        }
       }
-      flush_strip<G>(f, kind, idx, base, step, n, m, s_il, s_iu, lc, ls);
     }
     __syncthreads();
   }
 }
 
+// Applies the rotation log left by the sweep kernel: one launch per phase,
+// blockIdx.y = matrix, each thread one strip of phase `ph` (phase 0: 2n
+// strips, rows of Z then rows of T; phase 1: n strips, columns of T).  The
+// strips of a matrix are independent within a phase, so the threads never
+// synchronise.
+template <int G, int TC, int NT>
+__global__ __launch_bounds__(NT) void qr_apply_kernel(int n, ZMat Tm, ZMat Zm, QrLog log, int ph) {
+  __shared__ double tc[G * TC];
+  __shared__ zc ts[G * TC];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const int st = blockIdx.x * NT + tid;
+  const bool live = st < (ph == 0 ? 2 * n : n);
+  zc* T = Tm.p + (long long)b * Tm.stride;
+  zc* Z = Zm.p + (long long)b * Zm.stride;
+  const QrLogK LK = qr_log_k(log, b);
+  const QrLogM L = qr_log_of(log, b);
+  const int nsw = ((KInt)log.nsw)[b];
+  for (int g0 = 0; g0 < nsw; g0 += G) {
+    const SweepGroup<G> g = load_group<G>(LK, g0, min(G, nsw - g0));
+    zc* base = nullptr;
+    long long step = 0;
+    unsigned mask = 0;
+    if (live) mask = strip_of<G>(ph, st, n, g, T, Tm.ld, Z, Zm.ld, base, step);
+    if (!__syncthreads_or(mask != 0)) continue;
+    const int t0 = (g.tmin / (2 * G)) * (2 * G);
+    for (int T0 = t0; T0 <= g.tmax; T0 += TC) {
+      const int T1 = min(T0 + TC - 1, g.tmax);
+      __syncthreads();
+      fill_table<G, TC>(g, L, T0, ph, tc, ts, tid, NT);
+      __syncthreads();
+      if (mask == (1u << g.m) - 1u)
+        apply_chunk_strip<G, TC, false>(g, T0, T1, mask, base, step, tc, ts);
+      else if (mask)
+        apply_chunk_strip<G, TC, true>(g, T0, T1, mask, base, step, tc, ts);
+    }
+  }
+}
+
 template <int R, int NT, int G>
 __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm, int* info,
-                                                         int maxit_per_row, double* logc,
-                                                         zc* logs) {
+                                                         int maxit_per_row, QrLog log) {
   constexpr int LDB = R + 3;
   __shared__ zc band[LDB * LDB];
   __shared__ double rc[R];
   __shared__ zc rs[R];
-  __shared__ int s_il, s_lil[G], s_liu[G];
+  __shared__ int s_il;
   __shared__ zc s_mu;
+  __shared__ double s_tc[G * FLUSH_TC];
+  __shared__ zc s_ts[G * FLUSH_TC];
   const int b = blockIdx.x;
   zc* T = Tm.p + (long long)b * Tm.stride;
   zc* Z = Zm.p + (long long)b * Zm.stride;
-  double* lc = logc + (size_t)b * G * n;
-  zc* ls = logs + (size_t)b * G * n;
+  const QrLogM L = qr_log_of(log, b);
   const long long ld = Tm.ld, ldz = Zm.ld;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double eps = DBL_EPSILON;
   const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
-  int iu = n - 1, iter = 0, nlog = 0;
+  int iu = n - 1, iter = 0, nsw = 0, used = 0;
   long long total = 0;
-  while (iu > 0) {
+  bool failed = false;
+  for (;;) {
+    if (iu <= 0 || failed) break;
     if (tid == 0) s_il = 0;
     __syncthreads();
     for (int k = iu - tid; k >= 1; k -= NT) {
@@ -594,7 +781,16 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
     ++total;
     if (total > maxit) {
       if (tid == 0 && info) info[b] = 1;
-      break;
+      failed = true;
+      continue;
+    }
+    if (nsw == log.scap || (long long)used + (iu - il) > log.limit) {
+      // log full: apply it here (rare; the log holds about 2n sweeps)
+      __threadfence_block();
+      __syncthreads();
+      flush_log<G, FLUSH_TC>(n, nsw, L, T, ld, Z, ldz, tid, NT, s_tc, s_ts);
+      nsw = 0;
+      used = 0;
     }
     if (tid == 0) {
       if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
@@ -607,8 +803,9 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
         mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
                        T[iu + (iu - 1) * ld], T[iu + iu * ld]);
       s_mu = mu;
-      s_lil[nlog] = il;
-      s_liu[nlog] = iu;
+      L.il[nsw] = il;
+      L.iu[nsw] = iu;
+      L.off[nsw] = used;
     }
     __syncthreads();
     for (int k0 = il; k0 < iu;) {
@@ -672,8 +869,8 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
         T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
       }
       if (tid < nr) {
-        lc[(size_t)nlog * n + (k0 - il) + tid] = rc[tid];
-        ls[(size_t)nlog * n + (k0 - il) + tid] = rs[tid];
+        L.c[used + (k0 - il) + tid] = rc[tid];
+        L.s[used + (k0 - il) + tid] = rs[tid];
       }
       // rows k0..k1 of the active block right of the band
       for (int j = be + 1 + tid; j <= iu; j += NT) {
@@ -718,16 +915,10 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
       __syncthreads();
       k0 = k1;
     }
-    ++nlog;
-    if (nlog == G) {
-      __threadfence_block();
-      defer_flush<G>(n, nlog, s_lil, s_liu, lc, ls, T, ld, Z, ldz, tid, NT);
-      __syncthreads();
-      nlog = 0;
-    }
+    used += iu - il;
+    ++nsw;
   }
-  __syncthreads();
-  if (nlog > 0) defer_flush<G>(n, nlog, s_lil, s_liu, lc, ls, T, ld, Z, ldz, tid, NT);
+  if (tid == 0) log.nsw[b] = nsw;
 }
 
 __global__ __launch_bounds__(64) void trevc_kernel(int n, ZCMat Tm, ZMat Xm) {
@@ -777,12 +968,20 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch) {
 int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                              int maxit_per_row);
 
+namespace {
+long long qr_log_default(int n) { return 2LL * n * n + 64; }
+int qr_log_sweeps(int n) { return 8 * n + 64; }
+}  // namespace
+
 size_t schur_ws_bytes(int n, int batch) {
-  return (size_t)batch * SCHUR_G * n * (sizeof(double) + sizeof(zc)) + 512;
+  const size_t cap = (size_t)qr_log_default(n), scap = (size_t)qr_log_sweeps(n);
+  return (size_t)batch * (cap * (sizeof(double) + sizeof(zc)) + scap * 3 * sizeof(int) +
+                          sizeof(int)) +
+         6 * 256;
 }
 
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
-                  int maxit_per_row, void* ws) {
+                  int maxit_per_row, void* ws, long long log_cap) {
   if (n <= 0 || batch <= 0) return 0;
   // RCWA_QR_SWEEP=direct selects the unwindowed sweep, =window the windowed
   // sweep that updates all of T and Z at once (same arithmetic, same result).
@@ -792,10 +991,32 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
   }();
   if (mode == 'd') return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
   if (ws && mode != 'w') {
-    double* lc = (double*)ws;
-    zc* ls = (zc*)(((uintptr_t)(lc + (size_t)batch * SCHUR_G * n) + 255) & ~(uintptr_t)255);
+    auto align = [](uintptr_t p) { return (p + 255) & ~(uintptr_t)255; };
+    const long long cap_full = qr_log_default(n);
+    const int scap = qr_log_sweeps(n);
+    QrLog log;
+    log.c = (double*)ws;
+    log.s = (zc*)align((uintptr_t)(log.c + (size_t)batch * cap_full));
+    log.il = (int*)align((uintptr_t)(log.s + (size_t)batch * cap_full));
+    log.iu = (int*)align((uintptr_t)(log.il + (size_t)batch * scap));
+    log.off = (int*)align((uintptr_t)(log.iu + (size_t)batch * scap));
+    log.nsw = (int*)align((uintptr_t)(log.off + (size_t)batch * scap));
+    log.cap = cap_full;
+    log.scap = scap;
+    // a smaller limit (at least one sweep's worth) only makes the sweep kernel
+    // apply its log earlier
+    log.limit = log_cap > 0 ? std::max<long long>(std::min(log_cap, cap_full), n) : cap_full;
     hipLaunchKernelGGL((schur_defer_kernel<12, 256, SCHUR_G>), dim3(batch), dim3(256), 0, s, n,
-                       T, Z, info, maxit_per_row, lc, ls);
+                       T, Z, info, maxit_per_row, log);
+    int e = (int)hipGetLastError();
+    if (e) return e;
+    hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
+                       dim3((2 * n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T,
+                       Z, log, 0);
+    if ((e = (int)hipGetLastError())) return e;
+    hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
+                       dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T, Z,
+                       log, 1);
     return (int)hipGetLastError();
   }
   hipLaunchKernelGGL((schur_win_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z, info,
@@ -823,7 +1044,7 @@ size_t eig_workspace_bytes(int n, int batch) {
 }
 
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row, hipEvent_t* marks) {
+                int maxit_per_row, hipEvent_t* marks, long long log_cap) {
   if (n <= 0 || batch <= 0) return 0;
   zc* Zp = (zc*)ws;
   zc* Xp = Zp + (size_t)batch * n * n;
@@ -843,7 +1064,7 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   }
   if (marks) (void)hipEventRecord(marks[1], s);
   void* sws = (void*)(((uintptr_t)hws + hess_blocked_ws_bytes(n, batch) + 255) & ~(uintptr_t)255);
-  if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws))) return e;
+  if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws, log_cap))) return e;
   if (marks) (void)hipEventRecord(marks[2], s);
   if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch))) return e;
   hipLaunchKernelGGL(diag_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n,

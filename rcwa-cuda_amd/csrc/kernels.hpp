@@ -38,18 +38,21 @@ int zcount_nonfinite(hipStream_t s, const zc* p, long long count, int* flag);
 size_t eig_workspace_bytes(int n, int batch);
 // marks (optional): four events recorded before the Hessenberg reduction,
 // before the QR sweeps, before the eigenvector phase and at the end.
+// log_cap > 0 caps the QR rotation log per matrix (entries; default 2n^2):
+// when it fills, the sweep kernel applies it itself.  A tuning and test knob.
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row, hipEvent_t* marks = nullptr);
+                int maxit_per_row, hipEvent_t* marks = nullptr, long long log_cap = 0);
 // The three phases, exposed for tests.
 int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
 // Blocked (compact-WY panels + ZGEMM updates); ws holds hess_blocked_ws_bytes(n, batch).
 size_t hess_blocked_ws_bytes(int n, int batch);
 int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, void* ws);
-// ws (schur_ws_bytes) enables the deferred-update kernel; null selects the
-// windowed kernel that updates all of T and Z per window.
+// ws (schur_ws_bytes) enables the deferred-update kernels (sweeps on the
+// active block with a rotation log, then the log applied to Z and the rest of
+// T); null selects the windowed kernel that updates all of T and Z per window.
 size_t schur_ws_bytes(int n, int batch);
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row,
-                  void* ws = nullptr);
+                  void* ws = nullptr, long long log_cap = 0);
 // The same sweeps with every rotation applied directly in global memory.
 int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                              int maxit_per_row);

@@ -166,7 +166,7 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   CK(c.gemm(n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
   // eigendecomposition (rcwa.cpp:298-306)
   CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
-                 c.time_eig ? c.ev + 8 : nullptr));
+                 c.time_eig ? c.ev + 8 : nullptr, c.qr_log));
   if (c.time_eig) {
     float th = 0, ts = 0, tv = 0;
     CKH(hipEventSynchronize(c.ev[11]));
@@ -666,6 +666,12 @@ rcwa_status rcwa_ctx_destroy(rcwa_ctx* ctx) {
 rcwa_status rcwa_ctx_set_eig_maxit(rcwa_ctx* ctx, int maxit) {
   if (!ctx || maxit < 0) return rcwa_set_error(RCWA_E_ARG, "bad argument");
   reinterpret_cast<Ctx*>(ctx)->eig_maxit = maxit;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_set_qr_log(rcwa_ctx* ctx, long long rotations) {
+  if (!ctx || rotations < 0) return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  reinterpret_cast<Ctx*>(ctx)->qr_log = rotations;
   return RCWA_OK;
 }
 

@@ -277,7 +277,7 @@ rcwa_status rcwa_eig(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* A, rcwa_c
   if ((st = h2d(c, dA, A, blk))) return st;
   CKH(hipMemsetAsync(dinfo, 0, batch * sizeof(int), c.stream));
   CK(eig_batched(c.stream, n, ZMat{dA, n, (long long)n2}, dw, ZMat{dV, n, (long long)n2}, batch,
-                 ws, dinfo, c.eig_maxit));
+                 ws, dinfo, c.eig_maxit, nullptr, c.qr_log));
   std::vector<int> h(batch);
   if ((st = d2h(c, h.data(), dinfo, batch * sizeof(int)))) return st;
   for (int b = 0; b < batch; ++b)

@@ -25,6 +25,7 @@ struct Ctx {
   size_t split_cap = 0;
   size_t arena_cap = 0, arena_off = 0;
   int eig_maxit = 30;
+  long long qr_log = 0;  // QR rotation-log capacity per matrix (0 = default)
   int max_chunk = 0;
   rcwa_precision precision = RCWA_FP64;
   rcwa_stats stats{};

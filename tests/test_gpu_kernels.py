@@ -86,6 +86,29 @@ def test_eig_block_triangular(ctx, n, cuts):
         assert _match_eigs(wi, np.linalg.eigvals(m)) < 1e-10 * nrm
 
 
+@pytest.mark.parametrize("cap", [1, 3000])
+def test_eig_small_rotation_log(cap):
+    """A rotation log too small for the whole iteration: the sweep kernel
+    applies it itself whenever it fills (cap 1 is raised to one sweep's worth,
+    so that happens before nearly every sweep), and what is left is applied
+    afterwards.  Same eigenpairs as with the full log."""
+    n = 150
+    rng = np.random.default_rng(cap)
+    a = rnd(rng, n, n)
+    a[100:, :100] = 0       # a split in the middle as well
+    c = G.Context(0)
+    try:
+        c.set_qr_log(cap)
+        w, V = c.eig([a, a.T.copy()])
+    finally:
+        c.close()
+    for m, wi, vi in zip([a, a.T], w, V):
+        nrm = np.abs(m).max()
+        res = np.abs(m @ vi - vi * wi[None, :]).max() / nrm
+        assert res < 1e-12 * n, res
+        assert _match_eigs(wi, np.linalg.eigvals(m)) < 1e-10 * nrm
+
+
 def test_eig_workload_size(ctx):
     """n = 882: the Omega^2 size of 21 x 21 harmonics."""
     n = 882
