@@ -37,13 +37,16 @@ struct QrLog {
   int* iu;
   int* off;
   int* nsw;
+  int* state;    // per matrix: iu, sweeps on the current block, total sweeps, status
+  int* pending;  // matrices that stopped on a full log and must be resumed
   long long cap;
   long long limit;
   int scap;
 };
 
-constexpr int SCHUR_G = 4;     // sweeps per pass of the in-kernel fallback
-constexpr int FLUSH_TC = 128;  // time steps per LDS-staged chunk of that pass
+// status values of QrLog::state[4b + 3]
+constexpr int QR_RUNNING = 0, QR_DONE = 1, QR_FAILED = 2;
+
 constexpr int APPLY_G = 8;     // sweeps per pass of qr_apply_kernel
 constexpr int APPLY_TC = 128;  // time steps per LDS-staged chunk of qr_apply_kernel
 constexpr int APPLY_NT = 256;
@@ -661,44 +664,6 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
   }
 }
 
-// Applies the first nsw logged sweeps of this workgroup's matrix to Z and to
-// T outside each sweep's active block, G sweeps per pass: rows of Z and T
-// first, then columns of T.  A row strip and a column strip of T can share
-// entries; left and right rotations commute, but each strip needs its whole
-// row or column, so the two phases are separated by a barrier.  Called by all
-// nt threads of the workgroup: the fallback of the sweep kernel when its log
-// is full.  The log was written by this kernel, so it is read with ordinary
-// loads into the LDS table (tc, ts: G * TC entries).
-template <int G, int TC>
-__device__ void flush_log(int n, int nsw, const QrLogM& L, zc* T, long long ld, zc* Z,
-                          long long ldz, int tid, int nt, double* tc, zc* ts) {
-#pragma unroll 1
-  for (int ph = 0; ph < 2; ++ph) {
-    for (int g0 = 0; g0 < nsw; g0 += G) {
-      const SweepGroup<G> g = load_group<G>(L, g0, min(G, nsw - g0));
-      const int first = ph == 0 ? 0 : g.iumin + 1;
-      const int last = ph == 0 ? n + g.ilmax : n;  // exclusive
-      const int t0 = (g.tmin / (2 * G)) * (2 * G);
-      for (int T0 = t0; T0 <= g.tmax; T0 += TC) {
-        const int T1 = min(T0 + TC - 1, g.tmax);
-        __syncthreads();
-        fill_table<G, TC>(g, L, T0, ph, tc, ts, tid, nt);
-        __syncthreads();
-        for (int st = first + tid; st < last; st += nt) {
-          zc* base;
-          long long step;
-          const unsigned mask = strip_of<G>(ph, st, n, g, T, ld, Z, ldz, base, step);
-          if (mask == (1u << g.m) - 1u)
-            apply_chunk_strip<G, TC, false>(g, T0, T1, mask, base, step, tc, ts);
-          else if (mask)
-            apply_chunk_strip<G, TC, true>(g, T0, T1, mask, base, step, tc, ts);
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // Applies the rotation log left by the sweep kernel: one launch per phase,
 // blockIdx.y = matrix, each thread one strip of phase `ph` (phase 0: 2n
 // strips, rows of Z then rows of T; phase 1: n strips, columns of T).  The
@@ -737,17 +702,64 @@ __global__ __launch_bounds__(NT) void qr_apply_kernel(int n, ZMat Tm, ZMat Zm, Q
   }
 }
 
-template <int R, int NT, int G>
+// Applies a window's row rotations (rows k0..k0+nr) to one column: v points
+// at T[k0, j].
+template <int R>
+__device__ __forceinline__ void rotate_rows(zc* v, int nr, const double* rc, const zc* rs) {
+  zc x[R + 1];
+#pragma unroll
+  for (int q = 0; q <= R; ++q)
+    if (q <= nr) x[q] = v[q];
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (q < nr) {
+      const double c = rc[q];
+      const zc sr = rs[q], sc = zconj(sr);
+      const zc a = x[q], b = x[q + 1];
+      x[q] = zadd(zscale(a, c), zmul(sr, b));
+      x[q + 1] = zsub(zscale(b, c), zmul(sc, a));
+    }
+#pragma unroll
+  for (int q = 0; q <= R; ++q)
+    if (q <= nr) v[q] = x[q];
+}
+
+// Applies a window's column rotations (columns k0..k0+nr) to one row: v points
+// at T[i, k0], consecutive columns ld apart.
+template <int R>
+__device__ __forceinline__ void rotate_cols(zc* v, long long ld, int nr, const double* rc,
+                                            const zc* rs) {
+  zc x[R + 1];
+#pragma unroll
+  for (int q = 0; q <= R; ++q)
+    if (q <= nr) x[q] = v[q * ld];
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (q < nr) {
+      const double c = rc[q];
+      const zc sr = rs[q], sc = zconj(sr);
+      const zc a = x[q], b = x[q + 1];
+      x[q] = zadd(zscale(a, c), zmul(sc, b));
+      x[q + 1] = zsub(zscale(b, c), zmul(sr, a));
+    }
+#pragma unroll
+  for (int q = 0; q <= R; ++q)
+    if (q <= nr) v[q * ld] = x[q];
+}
+
+// One workgroup per matrix.  Resumes from the saved state and runs until
+// the matrix is in Schur form, the sweep limit is exceeded, or its rotation
+// log is full (then it saves its state, counts itself in *pending and stops;
+// the log is applied and the kernel launched again).
+template <int R, int NT>
 __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm, int* info,
                                                          int maxit_per_row, QrLog log) {
   constexpr int LDB = R + 3;
   __shared__ zc band[LDB * LDB];
-  __shared__ double rc[R];
-  __shared__ zc rs[R];
+  __shared__ double rc[2][R];  // rotations of the current and the previous window
+  __shared__ zc rs[2][R];
   __shared__ int s_il;
   __shared__ zc s_mu;
-  __shared__ double s_tc[G * FLUSH_TC];
-  __shared__ zc s_ts[G * FLUSH_TC];
   const int b = blockIdx.x;
   zc* T = Tm.p + (long long)b * Tm.stride;
   zc* Z = Zm.p + (long long)b * Zm.stride;
@@ -756,9 +768,14 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double eps = DBL_EPSILON;
   const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
-  int iu = n - 1, iter = 0, nsw = 0, used = 0;
-  long long total = 0;
-  bool failed = false;
+  int* st = log.state + 4 * b;
+  int iu = st[0], iter = st[1], nsw = 0, used = 0;
+  long long total = st[2];
+  if (st[3] != QR_RUNNING) {
+    if (tid == 0) log.nsw[b] = 0;
+    return;
+  }
+  bool failed = false, stopped = false;
   for (;;) {
     if (iu <= 0 || failed) break;
     if (tid == 0) s_il = 0;
@@ -777,20 +794,18 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
       __syncthreads();
       continue;
     }
+    if (nsw == log.scap || (long long)used + (iu - il) > log.limit) {
+      // log full (rare: it holds about 2n sweeps): stop before this sweep;
+      // the resumed kernel repeats the deflation test and carries on
+      stopped = true;
+      break;
+    }
     ++iter;
     ++total;
     if (total > maxit) {
       if (tid == 0 && info) info[b] = 1;
       failed = true;
       continue;
-    }
-    if (nsw == log.scap || (long long)used + (iu - il) > log.limit) {
-      // log full: apply it here (rare; the log holds about 2n sweeps)
-      __threadfence_block();
-      __syncthreads();
-      flush_log<G, FLUSH_TC>(n, nsw, L, T, ld, Z, ldz, tid, NT, s_tc, s_ts);
-      nsw = 0;
-      used = 0;
     }
     if (tid == 0) {
       if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
@@ -808,118 +823,135 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
       L.off[nsw] = used;
     }
     __syncthreads();
-    for (int k0 = il; k0 < iu;) {
-      const int k1 = min(k0 + R, iu);
+    // Windows of R rotations, software-pipelined: wave 0 chases the bulge
+    // through window w (band in LDS) while waves 1.. apply the rotations of
+    // window w-1 to the rest of the active block.  Window w-1's rows in the
+    // columns that band w adds (pbe+1 .. be) are updated by wave 0 itself
+    // before it loads band w, so the two groups never touch the same entries:
+    // wave 0 works on rows/columns b0..be, the other waves on columns right of
+    // be and on rows above pb0.
+    int pk0 = 0, pb0 = 0, pbe = 0, pnr = 0;
+    bool prev = false;
+    for (int k0 = il, w = 0;; ++w) {
+      const bool cur = k0 < iu;
+      const int k1 = cur ? min(k0 + R, iu) : k0;
       const int nr = k1 - k0;
       const int b0 = k0 > 0 ? k0 - 1 : 0;
       const int be = min(k1 + 1, n - 1);
       const int nb = be - b0 + 1;
-      for (int idx = tid; idx < nb * nb; idx += NT) {
-        const int i = idx % nb, j = idx / nb;
-        band[i + j * LDB] = T[(b0 + i) + (long long)(b0 + j) * ld];
-      }
-      __syncthreads();
+      const double* prc = rc[(w + 1) & 1];  // window w-1's rotations
+      const zc* prs = rs[(w + 1) & 1];
+      const int near_end = cur ? min(be, iu) : pbe;
       if (wave == 0) {
-        const int je = min(be, iu);
-        for (int k = k0; k < k1; ++k) {
-          double c = 0.0;
-          zc sr = zmake(0, 0);
-          if (lane == 0) {
-            zc f, g, r;
-            if (k == il) {
-              f = zsub(band[(k - b0) + (k - b0) * LDB], s_mu);
-              g = band[(k + 1 - b0) + (k - b0) * LDB];
-            } else {
-              f = band[(k - b0) + (k - 1 - b0) * LDB];
-              g = band[(k + 1 - b0) + (k - 1 - b0) * LDB];
-            }
-            givens(f, g, c, sr, r);
-            if (k > il) {
-              band[(k - b0) + (k - 1 - b0) * LDB] = r;
-              band[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
-            }
-            rc[k - k0] = c;
-            rs[k - k0] = sr;
-          }
-          c = __shfl(c, 0);
-          sr = zmake(__shfl(sr.x, 0), __shfl(sr.y, 0));
-          const zc sc = zconj(sr);
-          for (int j = k + lane; j <= je; j += 64) {
-            zc* p0 = &band[(k - b0) + (j - b0) * LDB];
-            const zc x = p0[0], y = p0[1];
-            p0[0] = zadd(zscale(x, c), zmul(sr, y));
-            p0[1] = zsub(zscale(y, c), zmul(sc, x));
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          const int ie = min(k + 2, iu);
-          for (int i = max(b0, il) + lane; i <= ie; i += 64) {
-            zc* q0 = &band[(i - b0) + (k - b0) * LDB];
-            const zc x = q0[0], y = q0[LDB];
-            q0[0] = zadd(zscale(x, c), zmul(sc, y));
-            q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (prev) {
+          for (int j = pbe + 1 + lane; j <= near_end; j += 64)
+            rotate_rows<R>(T + pk0 + (long long)j * ld, pnr, prc, prs);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
           __builtin_amdgcn_wave_barrier();
         }
+        if (cur) {
+          for (int idx = lane; idx < nb * nb; idx += 64) {
+            const int i = idx % nb, j = idx / nb;
+            band[i + j * LDB] = T[(b0 + i) + (long long)(b0 + j) * ld];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          double* crc = rc[w & 1];
+          zc* crs = rs[w & 1];
+          const int je = min(be, iu);
+          for (int k = k0; k < k1; ++k) {
+            double c = 0.0;
+            zc sr = zmake(0, 0);
+            if (lane == 0) {
+              zc f, g, r;
+              if (k == il) {
+                f = zsub(band[(k - b0) + (k - b0) * LDB], s_mu);
+                g = band[(k + 1 - b0) + (k - b0) * LDB];
+              } else {
+                f = band[(k - b0) + (k - 1 - b0) * LDB];
+                g = band[(k + 1 - b0) + (k - 1 - b0) * LDB];
+              }
+              givens(f, g, c, sr, r);
+              if (k > il) {
+                band[(k - b0) + (k - 1 - b0) * LDB] = r;
+                band[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
+              }
+              crc[k - k0] = c;
+              crs[k - k0] = sr;
+            }
+            c = __shfl(c, 0);
+            sr = zmake(__shfl(sr.x, 0), __shfl(sr.y, 0));
+            const zc sc = zconj(sr);
+            for (int j = k + lane; j <= je; j += 64) {
+              zc* p0 = &band[(k - b0) + (j - b0) * LDB];
+              const zc x = p0[0], y = p0[1];
+              p0[0] = zadd(zscale(x, c), zmul(sr, y));
+              p0[1] = zsub(zscale(y, c), zmul(sc, x));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int ie = min(k + 2, iu);
+            for (int i = max(b0, il) + lane; i <= ie; i += 64) {
+              zc* q0 = &band[(i - b0) + (k - b0) * LDB];
+              const zc x = q0[0], y = q0[LDB];
+              q0[0] = zadd(zscale(x, c), zmul(sc, y));
+              q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+          }
+          for (int idx = lane; idx < nb * nb; idx += 64) {
+            const int i = idx % nb, j = idx / nb;
+            T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
+          }
+          if (lane < nr) {
+            L.c[used + (k0 - il) + lane] = crc[lane];
+            L.s[used + (k0 - il) + lane] = crs[lane];
+          }
+        }
+      } else if (prev) {
+        const int t = tid - 64, nt = NT - 64;
+        // window w-1's rows right of what wave 0 covers
+        for (int j = near_end + 1 + t; j <= iu; j += nt)
+          rotate_rows<R>(T + pk0 + (long long)j * ld, pnr, prc, prs);
+        // window w-1's columns above its band
+        for (int i = il + t; i < pb0; i += nt)
+          rotate_cols<R>(T + i + (long long)pk0 * ld, ld, pnr, prc, prs);
       }
       __syncthreads();
-      for (int idx = tid; idx < nb * nb; idx += NT) {
-        const int i = idx % nb, j = idx / nb;
-        T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
-      }
-      if (tid < nr) {
-        L.c[used + (k0 - il) + tid] = rc[tid];
-        L.s[used + (k0 - il) + tid] = rs[tid];
-      }
-      // rows k0..k1 of the active block right of the band
-      for (int j = be + 1 + tid; j <= iu; j += NT) {
-        zc v[R + 1];
-        zc* col = T + k0 + (long long)j * ld;
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) v[q] = col[q];
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          if (q < nr) {
-            const double c = rc[q];
-            const zc sr = rs[q], sc = zconj(sr);
-            const zc x = v[q], y = v[q + 1];
-            v[q] = zadd(zscale(x, c), zmul(sr, y));
-            v[q + 1] = zsub(zscale(y, c), zmul(sc, x));
-          }
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) col[q] = v[q];
-      }
-      // columns k0..k1 of the active block above the band
-      for (int i = il + tid; i < b0; i += NT) {
-        zc* base = T + i + (long long)k0 * ld;
-        zc v[R + 1];
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) v[q] = base[q * ld];
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          if (q < nr) {
-            const double c = rc[q];
-            const zc sr = rs[q], sc = zconj(sr);
-            const zc x = v[q], y = v[q + 1];
-            v[q] = zadd(zscale(x, c), zmul(sc, y));
-            v[q + 1] = zsub(zscale(y, c), zmul(sr, x));
-          }
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) base[q * ld] = v[q];
-      }
-      __syncthreads();
+      if (!cur) break;
+      prev = true;
+      pk0 = k0;
+      pb0 = b0;
+      pbe = be;
+      pnr = nr;
       k0 = k1;
     }
     used += iu - il;
     ++nsw;
   }
-  if (tid == 0) log.nsw[b] = nsw;
+  if (tid == 0) {
+    log.nsw[b] = nsw;
+    st[0] = iu;
+    st[1] = iter;
+    st[2] = (int)total;
+    st[3] = stopped ? QR_RUNNING : failed ? QR_FAILED : QR_DONE;
+    if (stopped) atomicAdd(log.pending, 1);
+  }
 }
+
+__global__ void qr_state_init_kernel(int n, int batch, int* state, int* pending) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) *pending = 0;
+  if (b < batch) {
+    state[4 * b + 0] = n - 1;
+    state[4 * b + 1] = 0;
+    state[4 * b + 2] = 0;
+    state[4 * b + 3] = QR_RUNNING;
+  }
+}
+
+__global__ void qr_clear_pending_kernel(int* pending) { *pending = 0; }
 
 __global__ __launch_bounds__(64) void trevc_kernel(int n, ZCMat Tm, ZMat Xm) {
   extern __shared__ zc rhs[];
@@ -976,8 +1008,8 @@ int qr_log_sweeps(int n) { return 8 * n + 64; }
 size_t schur_ws_bytes(int n, int batch) {
   const size_t cap = (size_t)qr_log_default(n), scap = (size_t)qr_log_sweeps(n);
   return (size_t)batch * (cap * (sizeof(double) + sizeof(zc)) + scap * 3 * sizeof(int) +
-                          sizeof(int)) +
-         6 * 256;
+                          5 * sizeof(int)) +
+         8 * 256;
 }
 
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
@@ -1001,23 +1033,40 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
     log.iu = (int*)align((uintptr_t)(log.il + (size_t)batch * scap));
     log.off = (int*)align((uintptr_t)(log.iu + (size_t)batch * scap));
     log.nsw = (int*)align((uintptr_t)(log.off + (size_t)batch * scap));
+    log.state = (int*)align((uintptr_t)(log.nsw + batch));
+    log.pending = (int*)align((uintptr_t)(log.state + 4 * (size_t)batch));
     log.cap = cap_full;
     log.scap = scap;
     // a smaller limit (at least one sweep's worth) only makes the sweep kernel
-    // apply its log earlier
+    // stop for its log to be applied more often
     log.limit = log_cap > 0 ? std::max<long long>(std::min(log_cap, cap_full), n) : cap_full;
-    hipLaunchKernelGGL((schur_defer_kernel<12, 256, SCHUR_G>), dim3(batch), dim3(256), 0, s, n,
-                       T, Z, info, maxit_per_row, log);
+    hipLaunchKernelGGL(qr_state_init_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, n,
+                       batch, log.state, log.pending);
     int e = (int)hipGetLastError();
     if (e) return e;
-    hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
-                       dim3((2 * n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T,
-                       Z, log, 0);
-    if ((e = (int)hipGetLastError())) return e;
-    hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
-                       dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T, Z,
-                       log, 1);
-    return (int)hipGetLastError();
+    // Each round: sweeps until done or log full, then the log applied.  A
+    // round makes progress on every pending matrix (at least one sweep), and
+    // the sweep limit bounds the sweeps, so the loop ends.
+    for (;;) {
+      hipLaunchKernelGGL((schur_defer_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z,
+                         info, maxit_per_row, log);
+      if ((e = (int)hipGetLastError())) return e;
+      hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
+                         dim3((2 * n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n,
+                         T, Z, log, 0);
+      if ((e = (int)hipGetLastError())) return e;
+      hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
+                         dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T,
+                         Z, log, 1);
+      if ((e = (int)hipGetLastError())) return e;
+      int pending = 0;
+      if ((e = (int)hipMemcpyAsync(&pending, log.pending, sizeof(int), hipMemcpyDeviceToHost, s)))
+        return e;
+      if ((e = (int)hipStreamSynchronize(s))) return e;
+      if (pending == 0) return 0;
+      hipLaunchKernelGGL(qr_clear_pending_kernel, dim3(1), dim3(1), 0, s, log.pending);
+      if ((e = (int)hipGetLastError())) return e;
+    }
   }
   hipLaunchKernelGGL((schur_win_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z, info,
                      maxit_per_row);
