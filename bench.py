@@ -128,9 +128,9 @@ def serve_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lams", type=int, default=12, help="wavelengths per step")
+    ap.add_argument("--lams", type=int, default=48, help="wavelengths per step")
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve", type=int, default=0,

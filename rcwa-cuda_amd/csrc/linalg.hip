@@ -5,12 +5,18 @@
 // Gauss-Jordan in place, blocked by nb columns.  One panel step:
 //   1. the panel kernel runs nb unblocked Gauss-Jordan steps on the n x nb
 //      column panel: pivot search in the not-yet-pivoted rows, a row swap
-//      applied to all n columns, and elimination restricted to the panel;
+//      applied to all n columns, and elimination restricted to the panel
+//      (for n <= 1024 the panel lives in registers, one row per thread, and
+//      the swaps reach the other columns in a separate pass);
 //      afterwards the panel holds the nb columns G of the accumulated
 //      elementary transform E = I + (G - I_K) e_K^T;
 //   2. the other columns receive E as one rank-nb update,
 //      M(:, j) += (G - I_K) M(K, j)  for j outside the panel,
 //      which is a ZGEMM on the matrix cores.
+// For n <= 1024 the scheme has two levels: blocks of 64 columns are factored
+// as 16-column register panels whose updates stay inside the block, and the
+// rest of the matrix receives one rank-64 update per block (the block then
+// holds the columns of the composite transform of its four panels).
 // Row swaps are undone at the end as column swaps in reverse order, so the
 // result is A^-1 (not (PA)^-1).
 #include <algorithm>
@@ -19,8 +25,9 @@
 
 namespace {
 
-constexpr int GJ_NB = 32;
+constexpr int GJ_NB = 32;      // panel width of the global-memory panel kernel (n > 1024)
 constexpr int GJ_THREADS = 512;
+constexpr int GJ_BLOCK = 64;   // outer block of the two-level scheme (n <= 1024)
 
 struct ArgMax {
   double v;
@@ -103,23 +110,141 @@ __global__ __launch_bounds__(GJ_THREADS) void gj_panel_kernel(int n, int k0, int
   }
 }
 
-// Gp(i, r) = A(i, k0+r) - delta(i, k0+r);  Tm(r, j) = A(k0+r, j), 0 on the panel.
-__global__ void gj_prep_kernel(int n, int k0, int nb, ZCMat Mm, zc* Gp, zc* Tm) {
+// The same panel step with the panel in registers, for n <= REG_ROWS: thread
+// t owns row t of the n x REG_NB panel.  Pivot rows are exchanged between the
+// two owning threads through LDS and the swap is applied to the panel only;
+// gj_swap_kernel applies the panel's swaps to the other columns afterwards.
+// Same pivot choice and the same arithmetic per entry as gj_panel_kernel.
+constexpr int REG_NB = 16;
+constexpr int REG_ROWS = 1024;
+
+__global__ __launch_bounds__(REG_ROWS) void gj_panel_reg_kernel(int n, int k0, int nb, ZMat Mm,
+                                                                int* perm, int* info) {
+  __shared__ zc xrow[2][REG_NB];  // rows exchanged by a swap
+  __shared__ zc rowc[REG_NB];     // scaled pivot row
+  __shared__ ArgMax red[REG_ROWS / 64];
+  __shared__ int s_piv;
+  __shared__ zc s_inv;
+  const int b = blockIdx.x;
+  zc* A = Mm.p + (long long)b * Mm.stride;
+  const long long ld = Mm.ld;
+  int* pv = perm + (long long)b * n;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const bool own = t < n;
+  zc p[REG_NB];
+#pragma unroll
+  for (int j = 0; j < REG_NB; ++j)
+    p[j] = (own && j < nb) ? A[t + (long long)(k0 + j) * ld] : zmake(0, 0);
+#pragma unroll
+  for (int q = 0; q < REG_NB; ++q) {
+    if (q >= nb) continue;  // uniform: the last panel may be narrower
+    const int c = k0 + q;
+    // 1. pivot: largest |A(i,c)|^2 over rows c..n-1 (first index on ties)
+    ArgMax best{-1.0, n};
+    if (own && t >= c) best = ArgMax{zabs2(p[q]), t};
+    for (int off = 32; off > 0; off >>= 1) {
+      ArgMax o{__shfl_down(best.v, off), __shfl_down(best.i, off)};
+      best = better(best, o);
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    if (t == 0) {
+      ArgMax r = red[0];
+      for (int w = 1; w < REG_ROWS / 64; ++w) r = better(r, red[w]);
+      int pr = (r.i < n) ? r.i : c;
+      if (!(r.v > 0.0)) {
+        if (info) info[b] = 1;
+        pr = c;
+      }
+      s_piv = pr;
+      pv[c] = pr;
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    // 2. swap panel rows c and pr
+    if (pr != c) {
+      if (t == c || t == pr) {
+#pragma unroll
+        for (int j = 0; j < REG_NB; ++j) xrow[t == c ? 0 : 1][j] = p[j];
+      }
+      __syncthreads();
+      if (t == c || t == pr) {
+#pragma unroll
+        for (int j = 0; j < REG_NB; ++j) p[j] = xrow[t == c ? 1 : 0][j];
+      }
+    }
+    // 3. scaled pivot row
+    if (t == c) {
+      const zc inv = zinv(p[q]);
+      s_inv = inv;
+#pragma unroll
+      for (int j = 0; j < REG_NB; ++j) rowc[j] = zmul(p[j], inv);
+    }
+    __syncthreads();
+    const zc inv = s_inv;
+    // 4. eliminate column c from every other row of the panel
+    if (t == c) {
+#pragma unroll
+      for (int j = 0; j < REG_NB; ++j) p[j] = (j == q) ? inv : rowc[j];
+    } else if (own) {
+      const zc aic = p[q];
+#pragma unroll
+      for (int j = 0; j < REG_NB; ++j)
+        if (j != q) p[j] = zsub(p[j], zmul(aic, rowc[j]));
+      p[q] = zneg(zmul(aic, inv));
+    }
+    __syncthreads();
+  }
+  if (own) {
+#pragma unroll
+    for (int j = 0; j < REG_NB; ++j)
+      if (j < nb) A[t + (long long)(k0 + j) * ld] = p[j];
+  }
+}
+
+// Applies the row swaps of panel k0..k0+nb-1 (in order) to the columns
+// outside the panel; one thread per column.
+__global__ void gj_swap_kernel(int n, int k0, int nb, ZMat Mm, const int* perm) {
+  const int b = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || (j >= k0 && j < k0 + nb)) return;
+  zc* col = Mm.p + (long long)b * Mm.stride + (long long)j * Mm.ld;
+  const int* pv = perm + (long long)b * n;
+  for (int q = 0; q < nb; ++q) {
+    const int c = k0 + q, pr = pv[c];
+    if (pr != c) {
+      const zc x = col[c];
+      col[c] = col[pr];
+      col[pr] = x;
+    }
+  }
+}
+
+// Gp(i, r) = A(i, k0+r) - delta(i, k0+r) (n x nb);
+// Tm(r, j) = A(k0+r, j0+j) (nb x ncols, leading dimension nb), 0 where j0+j
+// lies in the panel.
+__global__ void gj_prep_kernel(int n, int k0, int nb, int j0, int ncols, ZCMat Mm, zc* Gp,
+                               zc* Tm) {
   const int b = blockIdx.y;
   const zc* A = Mm.p + (long long)b * Mm.stride;
   const long long ld = Mm.ld;
   zc* G = Gp + (long long)b * n * nb;
-  zc* T = Tm + (long long)b * n * nb;
+  zc* T = Tm + (long long)b * ncols * nb;
   const int kend = min(n, k0 + nb), w = kend - k0;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n * w;
-       idx += gridDim.x * blockDim.x) {
-    const int i = idx % n, r = idx / n;
-    zc g = A[i + (long long)(k0 + r) * ld];
-    if (i == k0 + r) g.x -= 1.0;
-    G[i + (long long)r * n] = g;
-    const int j = i;  // reuse the index pair for the row block: (r, j)
-    zc t = (j >= k0 && j < kend) ? zmake(0, 0) : A[(k0 + r) + (long long)j * ld];
-    T[r + (long long)j * w] = t;
+  const long long tot = (long long)max(n, ncols) * w;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < tot;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % max(n, ncols)), r = (int)(idx / max(n, ncols));
+    if (i < n) {
+      zc g = A[i + (long long)(k0 + r) * ld];
+      if (i == k0 + r) g.x -= 1.0;
+      G[i + (long long)r * n] = g;
+    }
+    if (i < ncols) {
+      const int j = j0 + i;
+      T[r + (long long)i * w] =
+          (j >= k0 && j < kend) ? zmake(0, 0) : A[(k0 + r) + (long long)j * ld];
+    }
   }
 }
 
@@ -222,26 +347,54 @@ inline dim3 grid_for(long long elems, int batch) {
 }  // namespace
 
 size_t gj_workspace_bytes(int n, int batch) {
-  const size_t nb = GJ_NB;
+  const size_t nb = std::max(GJ_NB, GJ_BLOCK);
   return (size_t)batch * (2 * n * nb * sizeof(zc) + n * sizeof(int)) + 256;
 }
 
 int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* info) {
   if (n <= 0 || batch <= 0) return 0;
   zc* Gp = (zc*)ws;
-  zc* Tm = Gp + (size_t)batch * n * GJ_NB;
-  int* perm = (int*)(Tm + (size_t)batch * n * GJ_NB);
-  for (int k0 = 0; k0 < n; k0 += GJ_NB) {
-    const int w = std::min(GJ_NB, n - k0);
-    hipLaunchKernelGGL(gj_panel_kernel, dim3(batch), dim3(GJ_THREADS), 0, s, n, k0, w, M, perm,
-                       info);
-    if (k0 + w >= n && k0 == 0) break;  // single panel covers the matrix
-    hipLaunchKernelGGL(gj_prep_kernel, grid_for((long long)n * w, batch), dim3(256), 0, s, n, k0,
-                       w, ZCMat{M.p, M.ld, M.stride}, Gp, Tm);
-    int e = zgemm_batched(s, n, n, w, zmake(1, 0), ZCMat{Gp, n, (long long)n * w},
-                          ZCMat{Tm, w, (long long)n * w}, zmake(1, 0),
-                          ZCMat{M.p, M.ld, M.stride}, M, batch);
-    if (e) return e;
+  zc* Tm = Gp + (size_t)batch * n * GJ_BLOCK;
+  int* perm = (int*)(Tm + (size_t)batch * n * GJ_BLOCK);
+  // M(:, j0:j0+ncols) += Gp * Tm for the panel k0..k0+w-1
+  auto update = [&](int k0, int w, int j0, int ncols) {
+    hipLaunchKernelGGL(gj_prep_kernel, grid_for((long long)std::max(n, ncols) * w, batch),
+                       dim3(256), 0, s, n, k0, w, j0, ncols, ZCMat{M.p, M.ld, M.stride}, Gp, Tm);
+    ZMat C{M.p + (long long)j0 * M.ld, M.ld, M.stride};
+    return zgemm_batched(s, n, ncols, w, zmake(1, 0), ZCMat{Gp, n, (long long)n * w},
+                         ZCMat{Tm, w, (long long)ncols * w}, zmake(1, 0),
+                         ZCMat{C.p, C.ld, C.stride}, C, batch);
+  };
+  if (n <= REG_ROWS) {
+    // Two levels: blocks of GJ_BLOCK columns, each factored as panels of
+    // REG_NB columns in registers with the updates kept inside the block;
+    // then one rank-GJ_BLOCK update of the rest of the matrix per block.
+    for (int K0 = 0; K0 < n; K0 += GJ_BLOCK) {
+      const int W = std::min(GJ_BLOCK, n - K0);
+      for (int k0 = K0; k0 < K0 + W; k0 += REG_NB) {
+        const int w = std::min(REG_NB, K0 + W - k0);
+        hipLaunchKernelGGL(gj_panel_reg_kernel, dim3(batch), dim3(REG_ROWS), 0, s, n, k0, w, M,
+                           perm, info);
+        hipLaunchKernelGGL(gj_swap_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n, k0,
+                           w, M, perm);
+        if (W > w) {
+          int e = update(k0, w, K0, W);
+          if (e) return e;
+        }
+      }
+      if (W < n) {
+        int e = update(K0, W, 0, n);
+        if (e) return e;
+      }
+    }
+  } else {
+    for (int k0 = 0; k0 < n; k0 += GJ_NB) {
+      const int w = std::min(GJ_NB, n - k0);
+      hipLaunchKernelGGL(gj_panel_kernel, dim3(batch), dim3(GJ_THREADS), 0, s, n, k0, w, M, perm,
+                         info);
+      int e = update(k0, w, 0, n);
+      if (e) return e;
+    }
   }
   hipLaunchKernelGGL(gj_unswap_kernel, dim3(batch), dim3(256), 0, s, n, M, perm);
   return (int)hipGetLastError();
