@@ -163,7 +163,11 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // P, Q (rcwa.cpp:264-285); Omega^2 = P Q (rcwa.cpp:288)
   CK(build_pq(s, N, erc, urc, ZCMat{w.erci, N, N2}, ZCMat{w.urci, N, N2}, Kx, Ky, M(b0), M(b1),
               Bl));
-  CK(c.gemm(n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
+  // Omega^2 feeds the eigensolver, which amplifies its perturbations (P Q
+  // cancels heavily), so it is formed in FP64 in both precision forms: a
+  // split-precision P Q alone moves T by ~3e-6 on the 7x7-harmonic single
+  // layer, the other split products together by ~3e-7.
+  CK(zgemm_batched(s, n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
   // eigendecomposition (rcwa.cpp:298-306)
   CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
                  c.time_eig ? c.ev + 8 : nullptr, c.qr_log));

@@ -1,21 +1,31 @@
-// Split-precision complex GEMM on the gfx950 matrix cores, the product used
-// by the split form of the path (PQ, W/V and the S-matrix / Redheffer
+// Split-precision complex GEMM on the gfx950 FP16 matrix cores, the product
+// used by the split form of the path (PQ, W/V and the S-matrix / Redheffer
 // multiplies); the eigensolver and every inverse stay FP64.
 //
 //   C = alpha * A * B + beta * D     (complex128 in and out, column-major)
 //
-// Ozaki-style splitting: every row of A and every column of B is scaled by a
-// power of two so that its largest real or imaginary part lies in [0.5, 1),
-// then each real number x is split into two FP16 slices
-//   x = x1 + 2^-11 x2 + O(2^-22),   x1 = fp16(x),  x2 = fp16((x - x1) 2^11).
-// A real product keeps the three significant slice products
-//   X Y ~= X1 Y1 + 2^-11 (X1 Y2 + X2 Y1)
-// on v_mfma_f32_16x16x16_f16 with FP32 accumulation (separate "hi" and "lo"
-// accumulators), and a complex product is four real ones, 12 MFMAs per
-// 16x16x16 step.  The FP32 accumulators cover SLAB = 64 consecutive k; each
-// slab's hi + 2^-11 lo is then added into FP64 accumulators, which the
-// epilogue unscales and combines with alpha/beta in FP64.  The normwise error
-// is about 2^-22 from the split plus sqrt(SLAB) 2^-24 from accumulation.
+// Ozaki scheme with fixed-point slices.  Every row of A and every column of B
+// is scaled by a power of two s so that its real and imaginary parts lie in
+// (-1, 1); each scaled real number u is then cut into three 9-bit integer
+// slices
+//   u = 2^-9 (a1 + 2^-9 a2 + 2^-18 a3) + r,   |a1| <= 512, |a2|, |a3| <= 256,
+//   |r| <= 2^-28,
+// all exact in FP16.  A real product keeps the slice products of order
+// 2^-18 and 2^-27 and above:
+//   u v ~= 2^-18 [ a1 b1 + 2^-9 (a1 b2 + a2 b1) + 2^-18 (a1 b3 + a2 b2 + a3 b1) ].
+// The leading group a1 b1 is a sum of integers of at most 2^18; within a slab
+// of 32 consecutive k the complex combination (two real products) stays below
+// 2^24, so v_mfma_f32_16x16x16_f16 accumulates it EXACTLY in FP32.  The other
+// two groups share a second FP32 accumulator (the 2^-18 group is fed with
+// slices pre-scaled by 2^-9); its rounding is 2^-24 of a term already 2^-9
+// down, i.e. 2^-33 of the result.  After every slab both accumulators are
+// added into FP64 accumulators; the epilogue unscales and combines with
+// alpha/beta in FP64.  Normwise error: about 2^-27 (the dropped slice
+// products and the slicing remainder), against 2^-21 for FP32 accumulation
+// of unaligned slices.
+//
+// A complex 16x16x16 step takes 24 MFMAs: 4 for the leading group and 20 for
+// the others.
 //
 // Operand layout after the split pass (per batch entry and slice): groups of
 // four consecutive k for one row (A) or one column (B), i.e.
@@ -30,15 +40,18 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int TILE = 64;
-// FP32 accumulation runs over SLAB consecutive k; slab sums are added in FP64.
-constexpr int SLAB = 64;
+// Leading-group FP32 accumulation runs over SLAB consecutive k (exact).
+constexpr int SLAB = 32;
+// Slices per real part: A keeps a1, a2, 2^-9 a2, 2^-9 a3; B keeps b1, b2, 2^-9 b3.
+constexpr int NSA = 4, NSB = 3;
 
 inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
 // rows = number of vectors to split (rows of A or columns of B), K = their
 // length.  Element (v, k) of batch b is at src + b*stride + v*vs + k*ks.
-// Output: 4 slices (re1, re2, im1, im2) of Vp x Kp halfs, and the power-of-two
-// scale 2^e of every vector.
+// Output: 2*ns slices (real part slices, then imaginary part slices) of
+// Vp x Kp halfs, and the power-of-two scale 2^e of every vector.
+template <int NS>
 __global__ void split_kernel(int rows, int K, const zc* src, long long stride, long long vs,
                              long long ks, int Vp, int Kp, _Float16* out, double* scale) {
   const int b = blockIdx.y;
@@ -46,36 +59,50 @@ __global__ void split_kernel(int rows, int K, const zc* src, long long stride, l
   if (v >= Vp) return;
   const zc* p = src + (long long)b * stride + (long long)v * vs;
   const size_t slice = (size_t)Vp * Kp;
-  _Float16* o = out + (size_t)b * 4 * slice;
+  _Float16* o = out + (size_t)b * 2 * NS * slice;
   double m = 0.0;
   if (v < rows)
     for (int k = 0; k < K; ++k) {
       const zc z = p[k * ks];
       m = fmax(m, fmax(fabs(z.x), fabs(z.y)));
     }
+  // 2^e > m, so the scaled parts lie in (-1, 1)
   int e = 0;
   if (m > 0.0) frexp(m, &e);
-  const double s = ldexp(1.0, -e);
   scale[(size_t)b * Vp + v] = ldexp(1.0, e);
+  const double s = ldexp(1.0, -e);
   for (int q = 0; q < Kp / 4; ++q) {
-    h4 r1, r2, i1, i2;
+    h4 out_s[2][NS];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int k = 4 * q + t;
       zc z = zmake(0, 0);
       if (v < rows && k < K) z = p[k * ks];
-      const double xr = z.x * s, xi = z.y * s;
-      const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
-      r1[t] = hr;
-      i1[t] = hi;
-      r2[t] = (_Float16)((xr - (double)hr) * 2048.0);
-      i2[t] = (_Float16)((xi - (double)hi) * 2048.0);
+      const double part[2] = {z.x * s, z.y * s};
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        // exact in FP64: scaling by powers of two and subtracting the rounded
+        // leading bits
+        const double t1 = part[c] * 512.0;
+        const double a1 = rint(t1);
+        const double t2 = (t1 - a1) * 512.0;
+        const double a2 = rint(t2);
+        const double a3 = rint((t2 - a2) * 512.0);
+        out_s[c][0][t] = (_Float16)a1;
+        out_s[c][1][t] = (_Float16)a2;
+        if (NS == 4) {
+          out_s[c][2][t] = (_Float16)(a2 * (1.0 / 512.0));
+          out_s[c][3][t] = (_Float16)(a3 * (1.0 / 512.0));
+        } else {
+          out_s[c][2][t] = (_Float16)(a3 * (1.0 / 512.0));
+        }
+      }
     }
     const size_t at = ((size_t)q * Vp + v) * 4;
-    *(h4*)(o + at) = r1;
-    *(h4*)(o + slice + at) = r2;
-    *(h4*)(o + 2 * slice + at) = i1;
-    *(h4*)(o + 3 * slice + at) = i2;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < NS; ++j) *(h4*)(o + (size_t)(c * NS + j) * slice + at) = out_s[c][j];
   }
 }
 
@@ -83,6 +110,8 @@ __device__ __forceinline__ f4 mf(h4 a, h4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
+// A slices: index c*NSA + j (c = 0 real, 1 imaginary; j = a1, a2, a2/512,
+// a3/512).  B slices: c*NSB + j (j = b1, b2, b3/512).
 __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int Mp, int Np,
                                                        const _Float16* As, const double* sA,
                                                        const _Float16* Bs, const double* sB,
@@ -94,60 +123,82 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, lk = lane >> 4;
   const size_t sa = (size_t)Mp * Kp, sb = (size_t)Np * Kp;
-  const _Float16* A0 = As + (size_t)b * 4 * sa;
-  const _Float16* B0 = Bs + (size_t)b * 4 * sb;
+  const _Float16* A0 = As + (size_t)b * 2 * NSA * sa;
+  const _Float16* B0 = Bs + (size_t)b * 2 * NSB * sb;
   const int r0 = tm * TILE + wm * 32, c0 = tn * TILE + wn * 32;
-  f4 hr[2][2], lr[2][2], hi_[2][2], lo_[2][2];
+  // g0: leading group (exact), g1: the 2^-9 groups; r/i: real/imaginary part
+  f4 g0r[2][2], g0i[2][2], g1r[2][2], g1i[2][2];
   double accR[2][2][4], accI[2][2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      hr[i][j] = lr[i][j] = hi_[i][j] = lo_[i][j] = f4{0, 0, 0, 0};
+      g0r[i][j] = g0i[i][j] = g1r[i][j] = g1i[i][j] = f4{0, 0, 0, 0};
 #pragma unroll
       for (int r = 0; r < 4; ++r) accR[i][j][r] = accI[i][j][r] = 0.0;
     }
-  const double lo_s = 1.0 / 2048.0;
+  const double lo_s = 1.0 / 512.0;
   for (int k0 = 0; k0 < Kp; k0 += 16) {
     const size_t q = (size_t)(k0 / 4 + lk);
-    h4 a[2][4], bb[2][4];
+    // ar[i][j]: real-part slice j of row tile i; ai: imaginary; nai: -ai
+    h4 ar[2][NSA], ai[2][NSA], nai[2][NSA], br[2][NSB], bi[2][NSB];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const size_t at = (q * Mp + r0 + i * 16 + li) * 4;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) a[i][s] = *(const h4*)(A0 + s * sa + at);
+      for (int s = 0; s < NSA; ++s) {
+        ar[i][s] = *(const h4*)(A0 + s * sa + at);
+        ai[i][s] = *(const h4*)(A0 + (NSA + s) * sa + at);
+        nai[i][s] = -ai[i][s];
+      }
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const size_t at = (q * Np + c0 + j * 16 + li) * 4;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) bb[j][s] = *(const h4*)(B0 + s * sb + at);
+      for (int s = 0; s < NSB; ++s) {
+        br[j][s] = *(const h4*)(B0 + s * sb + at);
+        bi[j][s] = *(const h4*)(B0 + (NSB + s) * sb + at);
+      }
     }
-    // 12 products per tile, ordered so that consecutive MFMAs update
-    // different accumulators (16 independent chains)
-    h4 nai[2][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      nai[i][0] = -a[i][2];
-      nai[i][1] = -a[i][3];
-    }
-#define SPLIT_STEP(ACC, X, Y)                 \
-  _Pragma("unroll") for (int i = 0; i < 2; ++i) \
-  _Pragma("unroll") for (int j = 0; j < 2; ++j) ACC[i][j] = mf(X, Y, ACC[i][j]);
-    SPLIT_STEP(hr, a[i][0], bb[j][0])
-    SPLIT_STEP(hi_, a[i][0], bb[j][2])
-    SPLIT_STEP(lr, a[i][0], bb[j][1])
-    SPLIT_STEP(lo_, a[i][0], bb[j][3])
-    SPLIT_STEP(hr, nai[i][0], bb[j][2])
-    SPLIT_STEP(hi_, a[i][2], bb[j][0])
-    SPLIT_STEP(lr, a[i][1], bb[j][0])
-    SPLIT_STEP(lo_, a[i][1], bb[j][2])
-    SPLIT_STEP(lr, nai[i][0], bb[j][3])
-    SPLIT_STEP(lo_, a[i][2], bb[j][1])
-    SPLIT_STEP(lr, nai[i][1], bb[j][2])
-    SPLIT_STEP(lo_, a[i][3], bb[j][0])
-#undef SPLIT_STEP
-    // promote the FP32 partial sums to FP64 every SLAB k-values
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        // leading group: Re += a1r b1r - a1i b1i, Im += a1r b1i + a1i b1r
+        g0r[i][j] = mf(ar[i][0], br[j][0], g0r[i][j]);
+        g0i[i][j] = mf(ar[i][0], bi[j][0], g0i[i][j]);
+        g0r[i][j] = mf(nai[i][0], bi[j][0], g0r[i][j]);
+        g0i[i][j] = mf(ai[i][0], br[j][0], g0i[i][j]);
+      }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        // 2^-9 group: a1 b2 + a2 b1; 2^-18 group (pre-scaled slices):
+        // a1 (b3/512) + (a2/512) b2 + (a3/512) b1
+        g1r[i][j] = mf(ar[i][0], br[j][1], g1r[i][j]);
+        g1i[i][j] = mf(ar[i][0], bi[j][1], g1i[i][j]);
+        g1r[i][j] = mf(ar[i][1], br[j][0], g1r[i][j]);
+        g1i[i][j] = mf(ar[i][1], bi[j][0], g1i[i][j]);
+        g1r[i][j] = mf(ar[i][0], br[j][2], g1r[i][j]);
+        g1i[i][j] = mf(ar[i][0], bi[j][2], g1i[i][j]);
+        g1r[i][j] = mf(ar[i][2], br[j][1], g1r[i][j]);
+        g1i[i][j] = mf(ar[i][2], bi[j][1], g1i[i][j]);
+        g1r[i][j] = mf(ar[i][3], br[j][0], g1r[i][j]);
+        g1i[i][j] = mf(ar[i][3], bi[j][0], g1i[i][j]);
+        g1r[i][j] = mf(nai[i][0], bi[j][1], g1r[i][j]);
+        g1i[i][j] = mf(ai[i][0], br[j][1], g1i[i][j]);
+        g1r[i][j] = mf(nai[i][1], bi[j][0], g1r[i][j]);
+        g1i[i][j] = mf(ai[i][1], br[j][0], g1i[i][j]);
+        g1r[i][j] = mf(nai[i][0], bi[j][2], g1r[i][j]);
+        g1i[i][j] = mf(ai[i][0], br[j][2], g1i[i][j]);
+        g1r[i][j] = mf(nai[i][2], bi[j][1], g1r[i][j]);
+        g1i[i][j] = mf(ai[i][2], br[j][1], g1i[i][j]);
+        g1r[i][j] = mf(nai[i][3], bi[j][0], g1r[i][j]);
+        g1i[i][j] = mf(ai[i][3], br[j][0], g1i[i][j]);
+      }
+    // promote to FP64 every SLAB k-values (keeps the leading group exact)
     if (((k0 + 16) % SLAB) == 0 || k0 + 16 >= Kp) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -155,10 +206,10 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            accR[i][j][r] += (double)hr[i][j][r] + lo_s * (double)lr[i][j][r];
-            accI[i][j][r] += (double)hi_[i][j][r] + lo_s * (double)lo_[i][j][r];
+            accR[i][j][r] += (double)g0r[i][j][r] + lo_s * (double)g1r[i][j][r];
+            accI[i][j][r] += (double)g0i[i][j][r] + lo_s * (double)g1i[i][j][r];
           }
-          hr[i][j] = lr[i][j] = hi_[i][j] = lo_[i][j] = f4{0, 0, 0, 0};
+          g0r[i][j] = g0i[i][j] = g1r[i][j] = g1i[i][j] = f4{0, 0, 0, 0};
         }
     }
   }
@@ -167,6 +218,7 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
   const bool use_d = (beta.x != 0.0 || beta.y != 0.0) && Dp != nullptr;
   const double* sAb = sA + (size_t)b * Mp;
   const double* sBb = sB + (size_t)b * Np;
+  const double unit = 1.0 / (512.0 * 512.0);  // the 2^-9 of each leading slice
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -177,7 +229,7 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
       for (int r = 0; r < 4; ++r) {
         const int row = r0 + i * 16 + 4 * lk + r;
         if (row >= M) continue;
-        const double sc = sAb[row] * sBb[col];
+        const double sc = sAb[row] * sBb[col] * unit;
         const zc prod = zmake(accR[i][j][r] * sc, accI[i][j][r] * sc);
         zc v = zmul(alpha, prod);
         if (use_d) v = zfma(beta, Dp[row + (long long)col * D.ld], v);
@@ -190,7 +242,8 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
 
 size_t zgemm_split_ws_bytes(int M, int N, int K, int batch) {
   const size_t Mp = rup(M, TILE), Np = rup(N, TILE), Kp = rup(K, 16);
-  return (size_t)batch * (4 * (Mp + Np) * Kp * sizeof(_Float16) + (Mp + Np) * sizeof(double)) +
+  return (size_t)batch *
+             (2 * (NSA * Mp + NSB * Np) * Kp * sizeof(_Float16) + (Mp + Np) * sizeof(double)) +
          1024;
 }
 
@@ -199,12 +252,13 @@ int zgemm_split_batched_ws(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A
   if (M <= 0 || N <= 0 || batch <= 0) return 0;
   const int Mp = rup(M, TILE), Np = rup(N, TILE), Kp = rup(K > 0 ? K : 1, 16);
   _Float16* As = (_Float16*)ws;
-  _Float16* Bs = As + (size_t)batch * 4 * Mp * Kp;
-  double* sA = (double*)(((uintptr_t)(Bs + (size_t)batch * 4 * Np * Kp) + 255) & ~(uintptr_t)255);
+  _Float16* Bs = As + (size_t)batch * 2 * NSA * Mp * Kp;
+  double* sA =
+      (double*)(((uintptr_t)(Bs + (size_t)batch * 2 * NSB * Np * Kp) + 255) & ~(uintptr_t)255);
   double* sB = sA + (size_t)batch * Mp;
-  hipLaunchKernelGGL(split_kernel, dim3((Mp + 255) / 256, batch), dim3(256), 0, s, M, K, A.p,
+  hipLaunchKernelGGL(split_kernel<NSA>, dim3((Mp + 255) / 256, batch), dim3(256), 0, s, M, K, A.p,
                      A.stride, 1LL, A.ld, Mp, Kp, As, sA);
-  hipLaunchKernelGGL(split_kernel, dim3((Np + 255) / 256, batch), dim3(256), 0, s, N, K, B.p,
+  hipLaunchKernelGGL(split_kernel<NSB>, dim3((Np + 255) / 256, batch), dim3(256), 0, s, N, K, B.p,
                      B.stride, B.ld, 1LL, Np, Kp, Bs, sB);
   const int tiles_n = Np / TILE, tiles_m = Mp / TILE;
   hipLaunchKernelGGL(split_mm_kernel, dim3(tiles_m * tiles_n, 1, batch), dim3(256), 0, s, M, N,
