@@ -604,17 +604,20 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
                                   long long step, const double* tc, const zc* ts) {
   constexpr int W = 2 * G;
   const int lo = g.lo, hi = g.hi, first = g.tmin - 2 * (G - 1);
-  // Slots that hold no column of the strip stay zero: identity rotations
-  // still combine them, and 1 * x + 0 * y must not meet garbage.
+  // Columns in flight at the chunk start: T0 - 2(G-1) .. T0.  T0 is a
+  // multiple of W, so column T0 - 2(G-1) + q sits in the static slot
+  // (q - 2(G-1)) mod W and the loads issue together.  Slots that hold no
+  // column of the strip stay zero: identity rotations still combine them, and
+  // 1 * x + 0 * y must not meet garbage.
   zc w[W];
+  const int cfirst = max(lo, max(T0, g.tmin) - 2 * (G - 1));
 #pragma unroll
-  for (int q = 0; q < W; ++q) w[q] = zmake(0, 0);
-  for (int c = max(lo, max(T0, g.tmin) - 2 * (G - 1)); c <= min(T0, hi); ++c) {
-    const zc v = base[(long long)c * step];
-#pragma unroll
-    for (int q = 0; q < W; ++q)
-      if ((c & (W - 1)) == q) w[q] = v;
+  for (int q = 0; q <= 2 * (G - 1); ++q) {
+    const int c = T0 - 2 * (G - 1) + q;
+    w[(q - 2 * (G - 1)) & (W - 1)] =
+        (c >= cfirst && c <= min(T0, hi)) ? base[(long long)c * step] : zmake(0, 0);
   }
+  w[1] = zmake(0, 0);  // the slot of column T0 + 1, which the first step loads
   int tb = T0;
   for (; tb <= T1; tb += W) {
     // The block's incoming columns tb+1 .. tb+W, loaded together.  The slot of
@@ -654,13 +657,12 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
         base[(long long)done * step] = w[(u - 2 * (G - 1)) & (W - 1)];
     }
   }
-  const int tend = tb - 1;  // last executed step; later columns are still in flight
-  for (int c = max(lo, tend - 2 * (G - 1) + 1); c <= min(tend + 1, hi); ++c) {
-    zc v = w[0];
+  // Still in flight after the last executed step tend = tb - 1 (tb is a
+  // multiple of W): columns tb - 2(G-1) .. tb, again in static slots.
 #pragma unroll
-    for (int q = 0; q < W; ++q)
-      if ((c & (W - 1)) == q) v = w[q];
-    base[(long long)c * step] = v;
+  for (int q = 0; q <= 2 * (G - 1); ++q) {
+    const int c = tb - 2 * (G - 1) + q;
+    if (c >= lo && c <= hi) base[(long long)c * step] = w[(q - 2 * (G - 1)) & (W - 1)];
   }
 }
 

@@ -166,7 +166,7 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // Omega^2 feeds the eigensolver, which amplifies its perturbations (P Q
   // cancels heavily), so it is formed in FP64 in both precision forms: a
   // split-precision P Q alone moves T by ~3e-6 on the 7x7-harmonic single
-  // layer, the other split products together by ~3e-7.
+  // layer, the other split products together by ~3e-7 (~1e-6 at 21x21).
   CK(zgemm_batched(s, n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
   // eigendecomposition (rcwa.cpp:298-306)
   CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
@@ -187,7 +187,8 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // lambda, X (rcwa.cpp:317, 339); V = Q W diag(1/lambda) (rcwa.cpp:326-330)
   CK(layer_modes(s, n, lam2, lam, ilam, X, k0t, Bl));
   CK(zscale_cols_to(s, n, n, CM(b3), ilam, n, M(b0), Bl));
-  CK(c.gemm(n, n, n, one, CM(b1), CM(b0), zero, none, M(b2), Bl));
+  // FP64 in both forms as well: V carries the eigenvectors into W^-1 and V^-1
+  CK(zgemm_batched(s, n, n, n, one, CM(b1), CM(b0), zero, none, M(b2), Bl));
   // W^-1 W0 = W^-1, V^-1 V0; A, B (rcwa.cpp:335-338)
   CK(gj_inverse_batched(s, n, M(b3), Bl, w.gj, info_sing));
   CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));

@@ -1,31 +1,34 @@
 // Split-precision complex GEMM on the gfx950 FP16 matrix cores, the product
-// used by the split form of the path (PQ, W/V and the S-matrix / Redheffer
-// multiplies); the eigensolver and every inverse stay FP64.
+// used by the split form of the path (the layer S-matrix and Redheffer
+// multiplies); Omega^2 = P Q, V = Q W, the eigensolver and every inverse stay
+// FP64.
 //
 //   C = alpha * A * B + beta * D     (complex128 in and out, column-major)
 //
 // Ozaki scheme with fixed-point slices.  Every row of A and every column of B
 // is scaled by a power of two s so that its real and imaginary parts lie in
-// (-1, 1); each scaled real number u is then cut into three 9-bit integer
+// (-1, 1); each scaled real number u is then cut into four 9-bit integer
 // slices
-//   u = 2^-9 (a1 + 2^-9 a2 + 2^-18 a3) + r,   |a1| <= 512, |a2|, |a3| <= 256,
-//   |r| <= 2^-28,
+//   u = 2^-9 (a1 + 2^-9 a2 + 2^-18 a3 + 2^-27 a4) + r,
+//   |a1| <= 512, |a2|, |a3|, |a4| <= 256, |r| <= 2^-37,
 // all exact in FP16.  A real product keeps the slice products of order
-// 2^-18 and 2^-27 and above:
-//   u v ~= 2^-18 [ a1 b1 + 2^-9 (a1 b2 + a2 b1) + 2^-18 (a1 b3 + a2 b2 + a3 b1) ].
-// The leading group a1 b1 is a sum of integers of at most 2^18; within a slab
-// of 32 consecutive k the complex combination (two real products) stays below
-// 2^24, so v_mfma_f32_16x16x16_f16 accumulates it EXACTLY in FP32.  The other
-// two groups share a second FP32 accumulator (the 2^-18 group is fed with
-// slices pre-scaled by 2^-9); its rounding is 2^-24 of a term already 2^-9
-// down, i.e. 2^-33 of the result.  After every slab both accumulators are
-// added into FP64 accumulators; the epilogue unscales and combines with
-// alpha/beta in FP64.  Normwise error: about 2^-27 (the dropped slice
-// products and the slicing remainder), against 2^-21 for FP32 accumulation
-// of unaligned slices.
+// 2^-36 and above:
+//   u v ~= 2^-18 [ a1 b1 + 2^-9 (a1 b2 + a2 b1) + 2^-18 (a1 b3 + a2 b2 + a3 b1)
+//                  + 2^-27 (a1 b4 + a2 b3 + a3 b2 + a4 b1) ].
+// The first two groups are sums of integers below 2^18 and 2^17; within a
+// slab of 32 consecutive k the complex combination (two real products)
+// keeps each below 2^24, so v_mfma_f32_16x16x16_f16 accumulates both
+// EXACTLY in FP32, each in its own accumulator.  The last two groups share a
+// third FP32 accumulator (the 2^-27 group fed with slices pre-scaled by
+// 2^-9); its rounding is 2^-24 of a term 2^-18 down, i.e. 2^-42 of the
+// result.  After every slab the accumulators are added into FP64
+// accumulators; the epilogue unscales and combines with alpha/beta in FP64.
+// Normwise error: about 2^-36 (the dropped slice products and the slicing
+// remainder).  Three slices (2^-27) were not enough: the layer S-matrix
+// products amplify their error about a hundredfold at 21x21 harmonics.
 //
-// A complex 16x16x16 step takes 24 MFMAs: 4 for the leading group and 20 for
-// the others.
+// A complex 16x16x16 step takes 40 MFMAs: 4 for the leading group, 8 for the
+// second and 28 for the last two.
 //
 // Operand layout after the split pass (per batch entry and slice): groups of
 // four consecutive k for one row (A) or one column (B), i.e.
@@ -40,10 +43,10 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int TILE = 64;
-// Leading-group FP32 accumulation runs over SLAB consecutive k (exact).
+// The exact FP32 accumulations run over SLAB consecutive k.
 constexpr int SLAB = 32;
-// Slices per real part: A keeps a1, a2, 2^-9 a2, 2^-9 a3; B keeps b1, b2, 2^-9 b3.
-constexpr int NSA = 4, NSB = 3;
+// Slices per real part: a1, a2, a3, 2^-9 a3, 2^-9 a4 (the same for B).
+constexpr int NS = 5;
 
 inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -51,7 +54,6 @@ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 // length.  Element (v, k) of batch b is at src + b*stride + v*vs + k*ks.
 // Output: 2*ns slices (real part slices, then imaginary part slices) of
 // Vp x Kp halfs, and the power-of-two scale 2^e of every vector.
-template <int NS>
 __global__ void split_kernel(int rows, int K, const zc* src, long long stride, long long vs,
                              long long ks, int Vp, int Kp, _Float16* out, double* scale) {
   const int b = blockIdx.y;
@@ -87,15 +89,14 @@ __global__ void split_kernel(int rows, int K, const zc* src, long long stride, l
         const double a1 = rint(t1);
         const double t2 = (t1 - a1) * 512.0;
         const double a2 = rint(t2);
-        const double a3 = rint((t2 - a2) * 512.0);
+        const double t3 = (t2 - a2) * 512.0;
+        const double a3 = rint(t3);
+        const double a4 = rint((t3 - a3) * 512.0);
         out_s[c][0][t] = (_Float16)a1;
         out_s[c][1][t] = (_Float16)a2;
-        if (NS == 4) {
-          out_s[c][2][t] = (_Float16)(a2 * (1.0 / 512.0));
-          out_s[c][3][t] = (_Float16)(a3 * (1.0 / 512.0));
-        } else {
-          out_s[c][2][t] = (_Float16)(a3 * (1.0 / 512.0));
-        }
+        out_s[c][2][t] = (_Float16)a3;
+        out_s[c][3][t] = (_Float16)(a3 * (1.0 / 512.0));
+        out_s[c][4][t] = (_Float16)(a4 * (1.0 / 512.0));
       }
     }
     const size_t at = ((size_t)q * Vp + v) * 4;
@@ -110,8 +111,8 @@ __device__ __forceinline__ f4 mf(h4 a, h4 b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
-// A slices: index c*NSA + j (c = 0 real, 1 imaginary; j = a1, a2, a2/512,
-// a3/512).  B slices: c*NSB + j (j = b1, b2, b3/512).
+// Slices: index c*NS + j (c = 0 real, 1 imaginary; j = a1, a2, a3, a3/512,
+// a4/512), for A and for B.
 __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int Mp, int Np,
                                                        const _Float16* As, const double* sA,
                                                        const _Float16* Bs, const double* sB,
@@ -123,32 +124,33 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, lk = lane >> 4;
   const size_t sa = (size_t)Mp * Kp, sb = (size_t)Np * Kp;
-  const _Float16* A0 = As + (size_t)b * 2 * NSA * sa;
-  const _Float16* B0 = Bs + (size_t)b * 2 * NSB * sb;
+  const _Float16* A0 = As + (size_t)b * 2 * NS * sa;
+  const _Float16* B0 = Bs + (size_t)b * 2 * NS * sb;
   const int r0 = tm * TILE + wm * 32, c0 = tn * TILE + wn * 32;
-  // g0: leading group (exact), g1: the 2^-9 groups; r/i: real/imaginary part
-  f4 g0r[2][2], g0i[2][2], g1r[2][2], g1i[2][2];
+  // g0: leading group, g1: the 2^-9 group (both exact), g2: the 2^-18 and
+  // 2^-27 groups; r/i: real/imaginary part
+  f4 g0r[2][2], g0i[2][2], g1r[2][2], g1i[2][2], g2r[2][2], g2i[2][2];
   double accR[2][2][4], accI[2][2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      g0r[i][j] = g0i[i][j] = g1r[i][j] = g1i[i][j] = f4{0, 0, 0, 0};
+      g0r[i][j] = g0i[i][j] = g1r[i][j] = g1i[i][j] = g2r[i][j] = g2i[i][j] = f4{0, 0, 0, 0};
 #pragma unroll
       for (int r = 0; r < 4; ++r) accR[i][j][r] = accI[i][j][r] = 0.0;
     }
-  const double lo_s = 1.0 / 512.0;
+  const double s1 = 1.0 / 512.0, s2 = 1.0 / (512.0 * 512.0);
   for (int k0 = 0; k0 < Kp; k0 += 16) {
     const size_t q = (size_t)(k0 / 4 + lk);
     // ar[i][j]: real-part slice j of row tile i; ai: imaginary; nai: -ai
-    h4 ar[2][NSA], ai[2][NSA], nai[2][NSA], br[2][NSB], bi[2][NSB];
+    h4 ar[2][NS], ai[2][NS], nai[2][NS], br[2][NS], bi[2][NS];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const size_t at = (q * Mp + r0 + i * 16 + li) * 4;
 #pragma unroll
-      for (int s = 0; s < NSA; ++s) {
+      for (int s = 0; s < NS; ++s) {
         ar[i][s] = *(const h4*)(A0 + s * sa + at);
-        ai[i][s] = *(const h4*)(A0 + (NSA + s) * sa + at);
+        ai[i][s] = *(const h4*)(A0 + (NS + s) * sa + at);
         nai[i][s] = -ai[i][s];
       }
     }
@@ -156,49 +158,34 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
     for (int j = 0; j < 2; ++j) {
       const size_t at = (q * Np + c0 + j * 16 + li) * 4;
 #pragma unroll
-      for (int s = 0; s < NSB; ++s) {
+      for (int s = 0; s < NS; ++s) {
         br[j][s] = *(const h4*)(B0 + s * sb + at);
-        bi[j][s] = *(const h4*)(B0 + (NSB + s) * sb + at);
+        bi[j][s] = *(const h4*)(B0 + (NS + s) * sb + at);
       }
     }
+    // one complex slice product x * y into accumulators (re, im)
+#define SPLIT_CMUL(RE, IM, XA, YB)                          \
+  RE[i][j] = mf(ar[i][XA], br[j][YB], RE[i][j]);            \
+  IM[i][j] = mf(ar[i][XA], bi[j][YB], IM[i][j]);            \
+  RE[i][j] = mf(nai[i][XA], bi[j][YB], RE[i][j]);           \
+  IM[i][j] = mf(ai[i][XA], br[j][YB], IM[i][j]);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        // leading group: Re += a1r b1r - a1i b1i, Im += a1r b1i + a1i b1r
-        g0r[i][j] = mf(ar[i][0], br[j][0], g0r[i][j]);
-        g0i[i][j] = mf(ar[i][0], bi[j][0], g0i[i][j]);
-        g0r[i][j] = mf(nai[i][0], bi[j][0], g0r[i][j]);
-        g0i[i][j] = mf(ai[i][0], br[j][0], g0i[i][j]);
+        SPLIT_CMUL(g0r, g0i, 0, 0)   // a1 b1
+        SPLIT_CMUL(g1r, g1i, 0, 1)   // a1 b2
+        SPLIT_CMUL(g1r, g1i, 1, 0)   // a2 b1
+        SPLIT_CMUL(g2r, g2i, 0, 2)   // a1 b3
+        SPLIT_CMUL(g2r, g2i, 1, 1)   // a2 b2
+        SPLIT_CMUL(g2r, g2i, 2, 0)   // a3 b1
+        SPLIT_CMUL(g2r, g2i, 0, 4)   // a1 (b4/512)
+        SPLIT_CMUL(g2r, g2i, 1, 3)   // a2 (b3/512)
+        SPLIT_CMUL(g2r, g2i, 3, 1)   // (a3/512) b2
+        SPLIT_CMUL(g2r, g2i, 4, 0)   // (a4/512) b1
       }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        // 2^-9 group: a1 b2 + a2 b1; 2^-18 group (pre-scaled slices):
-        // a1 (b3/512) + (a2/512) b2 + (a3/512) b1
-        g1r[i][j] = mf(ar[i][0], br[j][1], g1r[i][j]);
-        g1i[i][j] = mf(ar[i][0], bi[j][1], g1i[i][j]);
-        g1r[i][j] = mf(ar[i][1], br[j][0], g1r[i][j]);
-        g1i[i][j] = mf(ar[i][1], bi[j][0], g1i[i][j]);
-        g1r[i][j] = mf(ar[i][0], br[j][2], g1r[i][j]);
-        g1i[i][j] = mf(ar[i][0], bi[j][2], g1i[i][j]);
-        g1r[i][j] = mf(ar[i][2], br[j][1], g1r[i][j]);
-        g1i[i][j] = mf(ar[i][2], bi[j][1], g1i[i][j]);
-        g1r[i][j] = mf(ar[i][3], br[j][0], g1r[i][j]);
-        g1i[i][j] = mf(ar[i][3], bi[j][0], g1i[i][j]);
-        g1r[i][j] = mf(nai[i][0], bi[j][1], g1r[i][j]);
-        g1i[i][j] = mf(ai[i][0], br[j][1], g1i[i][j]);
-        g1r[i][j] = mf(nai[i][1], bi[j][0], g1r[i][j]);
-        g1i[i][j] = mf(ai[i][1], br[j][0], g1i[i][j]);
-        g1r[i][j] = mf(nai[i][0], bi[j][2], g1r[i][j]);
-        g1i[i][j] = mf(ai[i][0], br[j][2], g1i[i][j]);
-        g1r[i][j] = mf(nai[i][2], bi[j][1], g1r[i][j]);
-        g1i[i][j] = mf(ai[i][2], br[j][1], g1i[i][j]);
-        g1r[i][j] = mf(nai[i][3], bi[j][0], g1r[i][j]);
-        g1i[i][j] = mf(ai[i][3], br[j][0], g1i[i][j]);
-      }
-    // promote to FP64 every SLAB k-values (keeps the leading group exact)
+#undef SPLIT_CMUL
+    // promote to FP64 every SLAB k-values (keeps the first two groups exact)
     if (((k0 + 16) % SLAB) == 0 || k0 + 16 >= Kp) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -206,10 +193,13 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
         for (int j = 0; j < 2; ++j) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            accR[i][j][r] += (double)g0r[i][j][r] + lo_s * (double)g1r[i][j][r];
-            accI[i][j][r] += (double)g0i[i][j][r] + lo_s * (double)g1i[i][j][r];
+            accR[i][j][r] += (double)g0r[i][j][r] +
+                             (s1 * (double)g1r[i][j][r] + s2 * (double)g2r[i][j][r]);
+            accI[i][j][r] += (double)g0i[i][j][r] +
+                             (s1 * (double)g1i[i][j][r] + s2 * (double)g2i[i][j][r]);
           }
-          g0r[i][j] = g0i[i][j] = g1r[i][j] = g1i[i][j] = f4{0, 0, 0, 0};
+          g0r[i][j] = g0i[i][j] = g1r[i][j] = g1i[i][j] = g2r[i][j] = g2i[i][j] =
+              f4{0, 0, 0, 0};
         }
     }
   }
@@ -243,7 +233,7 @@ __global__ __launch_bounds__(256) void split_mm_kernel(int M, int N, int Kp, int
 size_t zgemm_split_ws_bytes(int M, int N, int K, int batch) {
   const size_t Mp = rup(M, TILE), Np = rup(N, TILE), Kp = rup(K, 16);
   return (size_t)batch *
-             (2 * (NSA * Mp + NSB * Np) * Kp * sizeof(_Float16) + (Mp + Np) * sizeof(double)) +
+             (2 * NS * (Mp + Np) * Kp * sizeof(_Float16) + (Mp + Np) * sizeof(double)) +
          1024;
 }
 
@@ -252,13 +242,13 @@ int zgemm_split_batched_ws(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A
   if (M <= 0 || N <= 0 || batch <= 0) return 0;
   const int Mp = rup(M, TILE), Np = rup(N, TILE), Kp = rup(K > 0 ? K : 1, 16);
   _Float16* As = (_Float16*)ws;
-  _Float16* Bs = As + (size_t)batch * 2 * NSA * Mp * Kp;
+  _Float16* Bs = As + (size_t)batch * 2 * NS * Mp * Kp;
   double* sA =
-      (double*)(((uintptr_t)(Bs + (size_t)batch * 2 * NSB * Np * Kp) + 255) & ~(uintptr_t)255);
+      (double*)(((uintptr_t)(Bs + (size_t)batch * 2 * NS * Np * Kp) + 255) & ~(uintptr_t)255);
   double* sB = sA + (size_t)batch * Mp;
-  hipLaunchKernelGGL(split_kernel<NSA>, dim3((Mp + 255) / 256, batch), dim3(256), 0, s, M, K, A.p,
+  hipLaunchKernelGGL(split_kernel, dim3((Mp + 255) / 256, batch), dim3(256), 0, s, M, K, A.p,
                      A.stride, 1LL, A.ld, Mp, Kp, As, sA);
-  hipLaunchKernelGGL(split_kernel<NSB>, dim3((Np + 255) / 256, batch), dim3(256), 0, s, N, K, B.p,
+  hipLaunchKernelGGL(split_kernel, dim3((Np + 255) / 256, batch), dim3(256), 0, s, N, K, B.p,
                      B.stride, B.ld, 1LL, Np, Kp, Bs, sB);
   const int tiles_n = Np / TILE, tiles_m = Mp / TILE;
   hipLaunchKernelGGL(split_mm_kernel, dim3(tiles_m * tiles_n, 1, batch), dim3(256), 0, s, M, N,

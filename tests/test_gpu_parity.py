@@ -101,9 +101,10 @@ def test_c1_lossless_energy(ctx):
 
 
 def test_split_form_error_vs_fp64(ctx):
-    """Split-precision form (FP16-sliced GEMMs, FP64 eigensolver and
-    inverses) against the FP64 form on the same structure; the error is
-    reported, and bounded by the 1e-6 target."""
+    """Split-precision form (Ozaki-sliced FP16 GEMMs for the layer S-matrix
+    and Redheffer products; FP64 Omega^2, V, eigensolver and inverses) against
+    the FP64 form on the same structure; the error is reported, and bounded by
+    the 1e-6 target."""
     p = W.c1_single(grid=128, h=3)
     f = ctx.solve([p])[0]
     s = ctx.solve([p], precision=G.RCWA_SPLIT)[0]
