@@ -33,6 +33,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
 
 METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
+# HBM bytes (read + written) of the QR phase per 882x882 layer matrix, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR kernels at 240
+# matrices per launch, scaled by the counters' reading on a copy of known size
+# (profiles/qr_phase_21x21_pmc_traffic.md).
+QR_PHASE_HBM_BYTES_PER_MATRIX = 4567.2e9 / 240
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -214,7 +219,10 @@ def main():
     roofline = {"kernel": "QR phase: schur_defer_kernel + 2 x qr_apply_kernel (complex "
                           "single-shift QR sweeps, Schur form and Schur vectors)",
                 "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "traffic": QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats,
+                "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE passes, "
+                                "copy-kernel calibrated)",
                 "avg_launch_ms": avg_ms, "matrices_per_launch": per_launch_mats}
     whole = st["flops_algorithmic"] / (st["ms_total"] * 1e-3) / 1e12 if st["ms_total"] else 0.0
 
