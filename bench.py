@@ -91,8 +91,10 @@ class Hip:
 def serve_bench(args):
     """Serving form on a prefix of the seeded Poisson stream (mean 2000
     req/s, harmonics {7,11,15,21,25}^2, 1-30 layers, 256x256 grids): each
-    request is submitted at its arrival time; then the same requests run
-    one per call through the plain path."""
+    request is submitted at its arrival time.  The plain per-call path's rate
+    is measured on the first --serve-percall requests of the same stream, run
+    one per call, which also checks that each of them gives the same result
+    alone as in the mix."""
     import threading
     import rcwa_amd as G
     import workloads as W
@@ -111,18 +113,28 @@ def serve_bench(args):
         c0 = srv.clock()
         th.start()
         th.join()
-        res = [srv.wait(t) for t in tickets]
+        res = []
+        for i, t in enumerate(tickets):
+            res.append(srv.wait(t))
+            if (i + 1) % 50 == 0:
+                print("served %d/%d" % (i + 1, len(tickets)), file=sys.stderr, flush=True)
     lat = np.array([r[3]["t_finish"] - r[3]["t_submit"] for r in res])
     first = min(r[3]["t_submit"] for r in res)
     last = max(r[3]["t_finish"] for r in res)
     served = len(res) / (last - first)
+    nsample = min(len(stream), args.serve_percall)
     ctx = G.Context(0)
     t1 = time.perf_counter()
-    alone = [ctx.solve([p])[0] for _, p in stream]
-    percall = len(stream) / (time.perf_counter() - t1)
+    alone = []
+    for i, (_, p) in enumerate(stream[:nsample]):
+        alone.append(ctx.solve([p])[0])
+        if (i + 1) % 8 == 0:
+            print("per-call %d/%d" % (i + 1, nsample), file=sys.stderr, flush=True)
+    percall = nsample / (time.perf_counter() - t1)
     ctx.close()
     dmax = max(float(np.abs(a[0] - r[0]).max()) for a, r in zip(alone, res))
-    return {"requests": len(res), "solves_per_s": served, "percall_solves_per_s": percall,
+    return {"requests": len(res), "percall_sample": nsample,
+            "solves_per_s": served, "percall_solves_per_s": percall,
             "speedup_vs_percall": served / percall,
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                            "p99": float(np.percentile(lat, 99) * 1e3)},
@@ -138,6 +150,8 @@ def main():
     ap.add_argument("--lams", type=int, default=48, help="wavelengths per step")
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-percall", type=int, default=24,
+                    help="requests of the stream timed one per call")
     ap.add_argument("--serve", type=int, default=0,
                     help="also run the serving form on this many requests of the stream")
     ap.add_argument("--cpu-sample", choices=["layer", "star"], help=argparse.SUPPRESS)

@@ -6,17 +6,18 @@
 //   1. the panel kernel runs nb unblocked Gauss-Jordan steps on the n x nb
 //      column panel: pivot search in the not-yet-pivoted rows, a row swap
 //      applied to all n columns, and elimination restricted to the panel
-//      (for n <= 1024 the panel lives in registers, one row per thread, and
-//      the swaps reach the other columns in a separate pass);
+//      (for n <= 4096 the panel lives in registers, one to four rows per
+//      thread, and the swaps reach the other columns in a separate pass);
 //      afterwards the panel holds the nb columns G of the accumulated
 //      elementary transform E = I + (G - I_K) e_K^T;
 //   2. the other columns receive E as one rank-nb update,
 //      M(:, j) += (G - I_K) M(K, j)  for j outside the panel,
 //      which is a ZGEMM on the matrix cores.
-// For n <= 1024 the scheme has two levels: blocks of 64 columns are factored
-// as 16-column register panels whose updates stay inside the block, and the
-// rest of the matrix receives one rank-64 update per block (the block then
-// holds the columns of the composite transform of its four panels).
+// For n <= 4096 the scheme has two levels: blocks of 64 columns are factored
+// as register panels (16, 8 or 4 columns) whose updates stay inside the
+// block, and the rest of the matrix receives one rank-64 update per block
+// (the block then holds the columns of the composite transform of its
+// panels).
 // Row swaps are undone at the end as column swaps in reverse order, so the
 // result is A^-1 (not (PA)^-1).
 #include <algorithm>
@@ -25,9 +26,9 @@
 
 namespace {
 
-constexpr int GJ_NB = 32;      // panel width of the global-memory panel kernel (n > 1024)
+constexpr int GJ_NB = 32;      // panel width of the global-memory panel kernel (n > 4096)
 constexpr int GJ_THREADS = 512;
-constexpr int GJ_BLOCK = 64;   // outer block of the two-level scheme (n <= 1024)
+constexpr int GJ_BLOCK = 64;   // outer block of the two-level scheme (n <= 4096)
 
 struct ArgMax {
   double v;
@@ -110,19 +111,21 @@ __global__ __launch_bounds__(GJ_THREADS) void gj_panel_kernel(int n, int k0, int
   }
 }
 
-// The same panel step with the panel in registers, for n <= REG_ROWS: thread
-// t owns row t of the n x REG_NB panel.  Pivot rows are exchanged between the
-// two owning threads through LDS and the swap is applied to the panel only;
+// The same panel step with the panel in registers: thread t of REG_THREADS
+// owns rows t, t + REG_THREADS, ... (RPT of them) of the n x NBP panel, so
+// n <= RPT * REG_THREADS.  Pivot rows are exchanged between the owning
+// threads through LDS and the swap is applied to the panel only;
 // gj_swap_kernel applies the panel's swaps to the other columns afterwards.
 // Same pivot choice and the same arithmetic per entry as gj_panel_kernel.
-constexpr int REG_NB = 16;
-constexpr int REG_ROWS = 1024;
+constexpr int REG_THREADS = 1024;
 
-__global__ __launch_bounds__(REG_ROWS) void gj_panel_reg_kernel(int n, int k0, int nb, ZMat Mm,
-                                                                int* perm, int* info) {
-  __shared__ zc xrow[2][REG_NB];  // rows exchanged by a swap
-  __shared__ zc rowc[REG_NB];     // scaled pivot row
-  __shared__ ArgMax red[REG_ROWS / 64];
+template <int RPT, int NBP>
+__global__ __launch_bounds__(REG_THREADS) void gj_panel_reg_kernel(int n, int k0, int nb,
+                                                                   ZMat Mm, int* perm,
+                                                                   int* info) {
+  __shared__ zc xrow[2][NBP];  // rows exchanged by a swap
+  __shared__ zc rowc[NBP];     // scaled pivot row
+  __shared__ ArgMax red[REG_THREADS / 64];
   __shared__ int s_piv;
   __shared__ zc s_inv;
   const int b = blockIdx.x;
@@ -130,18 +133,25 @@ __global__ __launch_bounds__(REG_ROWS) void gj_panel_reg_kernel(int n, int k0, i
   const long long ld = Mm.ld;
   int* pv = perm + (long long)b * n;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const bool own = t < n;
-  zc p[REG_NB];
+  zc p[RPT][NBP];
 #pragma unroll
-  for (int j = 0; j < REG_NB; ++j)
-    p[j] = (own && j < nb) ? A[t + (long long)(k0 + j) * ld] : zmake(0, 0);
+  for (int r = 0; r < RPT; ++r) {
+    const int row = t + r * REG_THREADS;
 #pragma unroll
-  for (int q = 0; q < REG_NB; ++q) {
+    for (int j = 0; j < NBP; ++j)
+      p[r][j] = (row < n && j < nb) ? A[row + (long long)(k0 + j) * ld] : zmake(0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < NBP; ++q) {
     if (q >= nb) continue;  // uniform: the last panel may be narrower
     const int c = k0 + q;
     // 1. pivot: largest |A(i,c)|^2 over rows c..n-1 (first index on ties)
     ArgMax best{-1.0, n};
-    if (own && t >= c) best = ArgMax{zabs2(p[q]), t};
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const int row = t + r * REG_THREADS;
+      if (row < n && row >= c) best = better(best, ArgMax{zabs2(p[r][q]), row});
+    }
     for (int off = 32; off > 0; off >>= 1) {
       ArgMax o{__shfl_down(best.v, off), __shfl_down(best.i, off)};
       best = better(best, o);
@@ -150,7 +160,7 @@ __global__ __launch_bounds__(REG_ROWS) void gj_panel_reg_kernel(int n, int k0, i
     __syncthreads();
     if (t == 0) {
       ArgMax r = red[0];
-      for (int w = 1; w < REG_ROWS / 64; ++w) r = better(r, red[w]);
+      for (int w = 1; w < REG_THREADS / 64; ++w) r = better(r, red[w]);
       int pr = (r.i < n) ? r.i : c;
       if (!(r.v > 0.0)) {
         if (info) info[b] = 1;
@@ -161,44 +171,64 @@ __global__ __launch_bounds__(REG_ROWS) void gj_panel_reg_kernel(int n, int k0, i
     }
     __syncthreads();
     const int pr = s_piv;
-    // 2. swap panel rows c and pr
+    // 2. swap panel rows c and pr (owned by threads c % REG_THREADS and
+    // pr % REG_THREADS, in their register rows c / REG_THREADS, pr / ...)
     if (pr != c) {
-      if (t == c || t == pr) {
 #pragma unroll
-        for (int j = 0; j < REG_NB; ++j) xrow[t == c ? 0 : 1][j] = p[j];
+      for (int r = 0; r < RPT; ++r) {
+        const int row = t + r * REG_THREADS;
+        if (row == c || row == pr) {
+#pragma unroll
+          for (int j = 0; j < NBP; ++j) xrow[row == c ? 0 : 1][j] = p[r][j];
+        }
       }
       __syncthreads();
-      if (t == c || t == pr) {
 #pragma unroll
-        for (int j = 0; j < REG_NB; ++j) p[j] = xrow[t == c ? 1 : 0][j];
+      for (int r = 0; r < RPT; ++r) {
+        const int row = t + r * REG_THREADS;
+        if (row == c || row == pr) {
+#pragma unroll
+          for (int j = 0; j < NBP; ++j) p[r][j] = xrow[row == c ? 1 : 0][j];
+        }
       }
     }
     // 3. scaled pivot row
-    if (t == c) {
-      const zc inv = zinv(p[q]);
-      s_inv = inv;
 #pragma unroll
-      for (int j = 0; j < REG_NB; ++j) rowc[j] = zmul(p[j], inv);
+    for (int r = 0; r < RPT; ++r) {
+      if (t + r * REG_THREADS == c) {
+        const zc inv = zinv(p[r][q]);
+        s_inv = inv;
+#pragma unroll
+        for (int j = 0; j < NBP; ++j) rowc[j] = zmul(p[r][j], inv);
+      }
     }
     __syncthreads();
     const zc inv = s_inv;
     // 4. eliminate column c from every other row of the panel
-    if (t == c) {
 #pragma unroll
-      for (int j = 0; j < REG_NB; ++j) p[j] = (j == q) ? inv : rowc[j];
-    } else if (own) {
-      const zc aic = p[q];
+    for (int r = 0; r < RPT; ++r) {
+      const int row = t + r * REG_THREADS;
+      if (row == c) {
 #pragma unroll
-      for (int j = 0; j < REG_NB; ++j)
-        if (j != q) p[j] = zsub(p[j], zmul(aic, rowc[j]));
-      p[q] = zneg(zmul(aic, inv));
+        for (int j = 0; j < NBP; ++j) p[r][j] = (j == q) ? inv : rowc[j];
+      } else if (row < n) {
+        const zc aic = p[r][q];
+#pragma unroll
+        for (int j = 0; j < NBP; ++j)
+          if (j != q) p[r][j] = zsub(p[r][j], zmul(aic, rowc[j]));
+        p[r][q] = zneg(zmul(aic, inv));
+      }
     }
     __syncthreads();
   }
-  if (own) {
 #pragma unroll
-    for (int j = 0; j < REG_NB; ++j)
-      if (j < nb) A[t + (long long)(k0 + j) * ld] = p[j];
+  for (int r = 0; r < RPT; ++r) {
+    const int row = t + r * REG_THREADS;
+    if (row < n) {
+#pragma unroll
+      for (int j = 0; j < NBP; ++j)
+        if (j < nb) A[row + (long long)(k0 + j) * ld] = p[r][j];
+    }
   }
 }
 
@@ -365,16 +395,29 @@ int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* i
                          ZCMat{Tm, w, (long long)ncols * w}, zmake(1, 0),
                          ZCMat{C.p, C.ld, C.stride}, C, batch);
   };
-  if (n <= REG_ROWS) {
+  // Register panels: one row per thread up to n = 1024 (16-column panels),
+  // two up to 2048 (8 columns), four up to 4096 (4 columns).
+  int rpt = 0, nbp = 0;
+  if (n <= REG_THREADS) { rpt = 1; nbp = 16; }
+  else if (n <= 2 * REG_THREADS) { rpt = 2; nbp = 8; }
+  else if (n <= 4 * REG_THREADS) { rpt = 4; nbp = 4; }
+  if (rpt) {
     // Two levels: blocks of GJ_BLOCK columns, each factored as panels of
-    // REG_NB columns in registers with the updates kept inside the block;
-    // then one rank-GJ_BLOCK update of the rest of the matrix per block.
+    // nbp columns in registers with the updates kept inside the block; then
+    // one rank-GJ_BLOCK update of the rest of the matrix per block.
     for (int K0 = 0; K0 < n; K0 += GJ_BLOCK) {
       const int W = std::min(GJ_BLOCK, n - K0);
-      for (int k0 = K0; k0 < K0 + W; k0 += REG_NB) {
-        const int w = std::min(REG_NB, K0 + W - k0);
-        hipLaunchKernelGGL(gj_panel_reg_kernel, dim3(batch), dim3(REG_ROWS), 0, s, n, k0, w, M,
-                           perm, info);
+      for (int k0 = K0; k0 < K0 + W; k0 += nbp) {
+        const int w = std::min(nbp, K0 + W - k0);
+        if (rpt == 1)
+          hipLaunchKernelGGL((gj_panel_reg_kernel<1, 16>), dim3(batch), dim3(REG_THREADS), 0, s,
+                             n, k0, w, M, perm, info);
+        else if (rpt == 2)
+          hipLaunchKernelGGL((gj_panel_reg_kernel<2, 8>), dim3(batch), dim3(REG_THREADS), 0, s,
+                             n, k0, w, M, perm, info);
+        else
+          hipLaunchKernelGGL((gj_panel_reg_kernel<4, 4>), dim3(batch), dim3(REG_THREADS), 0, s,
+                             n, k0, w, M, perm, info);
         hipLaunchKernelGGL(gj_swap_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n, k0,
                            w, M, perm);
         if (W > w) {

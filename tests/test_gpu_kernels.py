@@ -37,6 +37,20 @@ def test_inverse(ctx, n):
         assert np.abs(x - np.linalg.inv(a)).max() / np.abs(x).max() < 1e-12
 
 
+@pytest.mark.parametrize("n", [1100, 2100])
+def test_inverse_large(ctx, n):
+    """Register panels with two (n <= 2048) and four (n <= 4096) rows per
+    thread; n = 1250 is the 25x25-harmonic size of the serving stream."""
+    rng = np.random.default_rng(n)
+    A = [rnd(rng, n, n) + 2 * np.sqrt(n) * np.eye(n) for _ in range(2)]
+    A[1][[0, 5]] = A[1][[5, 0]]   # force a swap at the first pivot
+    A[1][0, 0] = 0
+    X = ctx.inverse(A)
+    for a, x in zip(A, X):
+        assert np.abs(a @ x - np.eye(n)).max() < 1e-12 * n
+        assert np.abs(x - np.linalg.inv(a)).max() / np.abs(x).max() < 1e-11
+
+
 def test_inverse_needs_pivoting(ctx):
     # zero leading entry forces a row swap at the first step
     A = np.array([[0, 1, 2], [3, 4, 5], [6, 7, 9]], np.complex128) + 0.5j * np.eye(3)
