@@ -465,51 +465,85 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     if (st) return st;
   }
   CKH(hipEventRecord(c.ev[4], s));
-  // ---- fold (main.cpp:143-164), ends (main.cpp:166-170)
-  zc* S[4] = {slot[0], slot[1], slot[2], slot[3]};
-  zc* E[4] = {slot[4], slot[5], slot[6], slot[7]};
-  zc* T[5] = {slot[8], slot[9], slot[10], slot[11], slot[12]};
-  // SMatrixInit (rcwa.cpp:237-250) for every job; then, for the jobs with
-  // layers, SMatrixInit * S_0 = S_0 exactly in IEEE arithmetic (every
-  // product is with I or 0), so their fold starts from S_0.
-  CK(zset_identity(s, n, ZMat{S[0], n, n2}, G, zmake(0, 0)));
-  CK(zset_identity(s, n, ZMat{S[1], n, n2}, G, zmake(1, 0)));
-  CK(zset_identity(s, n, ZMat{S[2], n, n2}, G, zmake(1, 0)));
-  CK(zset_identity(s, n, ZMat{S[3], n, n2}, G, zmake(0, 0)));
-  if (Lmax > 0) {
-    const int a0 = act[0];
-    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, n2}, ZMat{S[0], n, n2}, a0));
-    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, n2}, ZMat{S[1], n, n2}, a0));
-    CK(zcopy_batched(s, n, n, ZCMat{S12l, n, n2}, ZMat{S[2], n, n2}, a0));
-    CK(zcopy_batched(s, n, n, ZCMat{S11l, n, n2}, ZMat{S[3], n, n2}, a0));
-    for (int l = 1; l < Lmax; ++l) {
-      const zc* b11 = S11l + (size_t)off[l] * n2;
-      const zc* b12 = S12l + (size_t)off[l] * n2;
-      const ZCMat B[4] = {{b11, n, n2}, {b12, n, n2}, {b12, n, n2}, {b11, n, n2}};
-      st = star_product(c, n, act[l], S, B, T, gjf, info_sing, act[l] < G);
-      if (st) return st;
+  // ---- fold (main.cpp:143-170) for the two source columns
+  // S_glob = S_ref * S_0 * ... * S_{L-1} * S_trn (Redheffer products,
+  // rcwa.cpp:588-615), associated left to right.  The efficiencies need only
+  // the columns of S_glob's S11 and S21 at the two source entries
+  // (rcwa.cpp:512-514, 535, 553), so the running product P keeps P12 and P22
+  // as matrices and only those columns a = P11 c, b = P21 c.  With
+  // F = (I - X11 P22)^-1, one step P * X is
+  //   a += P12 F X11 b,   b = X21 (b + P22 F X11 b),
+  //   P12 = P12 F X12,    P22 = X22 + X21 P22 F X12
+  // (the push-through identity (I - P22 X11)^-1 P22 = P22 F turns the
+  // reference's second inverse into products): one inverse and five n^3
+  // products per layer instead of two and ten, and the last step (S_trn)
+  // needs only the columns.
+  zc* P12 = slot[0];
+  zc* P22 = slot[1];
+  zc* P12n = slot[2];
+  zc* P22n = slot[3];
+  zc* Fm = slot[4];
+  zc* Gm = slot[5];
+  zc* Hm = slot[6];
+  zc* Id = slot[7];
+  zc* vu = slot[8];  // three n x 2 column pairs per job (a slot holds G n^2 >= G 6n)
+  zc* vw = vu + (size_t)G * 2 * n;
+  zc* vy = vw + (size_t)G * 2 * n;
+  zc* X11 = slot[9];
+  zc* X21 = slot[10];
+  zc* va = dcols;  // S11 columns, then S21 columns: read back by the host
+  zc* vb = dcols + (size_t)G * 2 * n;
+  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const ZCMat none{nullptr, 0, 0};
+  const ZCMat ident{Id, n, 0};  // stride 0: the same identity for every job
+  auto Mt = [&](zc* p) { return ZMat{p, n, n2}; };
+  auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  auto Mv = [&](zc* p) { return ZMat{p, n, 2LL * n}; };
+  auto CMv = [&](const zc* p) { return ZCMat{p, n, 2LL * n}; };
+  CK(zset_identity(s, n, ZMat{Id, n, n2}, 1, one));
+  // P = S_ref: full S12, S22; columns of S11, S21
+  CK(expand_block_diag(s, N, dends + 0 * N, 32LL * N, Mt(X11), G));
+  CK(expand_block_diag(s, N, dends + 4 * (size_t)N, 32LL * N, Mt(P12), G));
+  CK(expand_block_diag(s, N, dends + 8 * (size_t)N, 32LL * N, Mt(X21), G));
+  CK(expand_block_diag(s, N, dends + 12 * (size_t)N, 32LL * N, Mt(P22), G));
+  CK(gather_cols(s, n, CMt(X11), dcolidx, 2, va, G));
+  CK(gather_cols(s, n, CMt(X21), dcolidx, 2, vb, G));
+  // P <- P * X for the first na jobs
+  auto step = [&](int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22,
+                  bool columns_only) -> rcwa_status {
+    CK(c.gemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
+    CK(gj_inverse_batched(s, n, Mt(Fm), na, gjf, info_sing));
+    // the columns, with the old P12 and P22
+    CK(c.gemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
+    CK(c.gemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
+    CK(c.gemm(n, 2, n, one, CMt(P12), CMv(vw), one, CMv(va), Mv(va), na));
+    CK(c.gemm(n, 2, n, one, CMt(P22), CMv(vw), one, CMv(vb), Mv(vy), na));
+    CK(c.gemm(n, 2, n, one, x21, CMv(vy), zero, none, Mv(vb), na));
+    if (columns_only) return RCWA_OK;
+    CK(c.gemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
+    CK(c.gemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
+    CK(c.gemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
+    CK(c.gemm(n, n, n, one, x21, CMt(Hm), one, x22, Mt(P22n), na));
+    if (na < G) {  // the jobs already past their last layer keep their P
+      CK(zcopy_batched(s, n, n, CMt(P12 + (size_t)na * n2), Mt(P12n + (size_t)na * n2), G - na));
+      CK(zcopy_batched(s, n, n, CMt(P22 + (size_t)na * n2), Mt(P22n + (size_t)na * n2), G - na));
     }
-  }
-  // per-job end diagonals: [job][ref S11..S22, trn S11..S22][4][N]
-  for (int k = 0; k < 4; ++k)
-    CK(expand_block_diag(s, N, dends + (size_t)4 * k * N, 32LL * N, ZMat{E[k], n, n2}, G));
-  {
-    const ZCMat B[4] = {{S[0], n, n2}, {S[1], n, n2}, {S[2], n, n2}, {S[3], n, n2}};
-    st = star_product(c, n, G, E, B, T, gjf, info_sing, false);
+    std::swap(P12, P12n);
+    std::swap(P22, P22n);
+    return RCWA_OK;
+  };
+  for (int l = 0; l < Lmax; ++l) {
+    const ZCMat x11 = CMt(S11l + (size_t)off[l] * n2), x12 = CMt(S12l + (size_t)off[l] * n2);
+    st = step(act[l], x11, x12, x12, x11, false);  // layers: S21 = S12, S22 = S11
     if (st) return st;
   }
-  for (int k = 0; k < 4; ++k)
-    CK(expand_block_diag(s, N, dends + 16 * (size_t)N + (size_t)4 * k * N, 32LL * N,
-                         ZMat{S[k], n, n2}, G));
-  {
-    const ZCMat B[4] = {{S[0], n, n2}, {S[1], n, n2}, {S[2], n, n2}, {S[3], n, n2}};
-    st = star_product(c, n, G, E, B, T, gjf, info_sing, false);
-    if (st) return st;
-  }
+  // * S_trn: only the columns
+  CK(expand_block_diag(s, N, dends + 16 * (size_t)N, 32LL * N, Mt(X11), G));
+  CK(expand_block_diag(s, N, dends + 24 * (size_t)N, 32LL * N, Mt(X21), G));
+  st = step(G, CMt(X11), CMt(X11), CMt(X21), CMt(X11), true);
+  if (st) return st;
   CKH(hipEventRecord(c.ev[5], s));
   // ---- S11 / S21 columns at the source entries (rcwa.cpp:512-514, 535, 553)
-  CK(gather_cols(s, n, ZCMat{E[0], n, n2}, dcolidx, 2, dcols, G));
-  CK(gather_cols(s, n, ZCMat{E[2], n, n2}, dcolidx, 2, dcols + (size_t)G * 2 * n, G));
   std::vector<cplx> hcols((size_t)G * 4 * n);
   CKH(hipMemcpyAsync(hcols.data(), dcols, hcols.size() * sizeof(zc), hipMemcpyDeviceToHost, s));
   std::vector<int> hinfo(2 * T1);
