@@ -111,7 +111,7 @@ rcwa_status rcwa_ctx_destroy(rcwa_ctx* ctx);
 /* QR sweep limit per eigenvalue (default 30, as Eigen's ComplexSchur). */
 rcwa_status rcwa_ctx_set_eig_maxit(rcwa_ctx* ctx, int maxit_per_row);
 /* Rotation-log capacity of the QR iteration, in rotations per matrix
- * (0 = default, 2n^2).  The QR sweeps update only the active block and log
+ * (0 = default, n^2).  The QR sweeps update only the active block and log
  * their rotations; the log is applied to the Schur vectors and the rest of
  * the Schur form after the iteration, or earlier when it fills up.  A tuning
  * and test knob with no counterpart in the reference (Eigen's ComplexSchur,

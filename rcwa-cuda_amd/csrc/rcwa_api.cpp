@@ -648,14 +648,15 @@ rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_prec
     size_t i = 0;
     while (i < v.size()) {
       // largest prefix (at most max_chunk jobs) whose chunk fits the budget
-      // with a one-layer sub-batch
+      // with a layer sub-batch of min(its layers, kMinLayerBatch)
       size_t G = v.size() - i;
       if (c.max_chunk > 0) G = std::min(G, (size_t)c.max_chunk);
       for (;;) {
         int TL = 0;
         for (size_t k = i; k < i + G; ++k) TL += v[k]->dev->num_layers;
-        const size_t need = chunk_bytes(N, TL, v[i]->dev->num_layers, j0.dev->Nx, j0.dev->Ny, hx,
-                                        hy, !j0.dev->grids_on_device, (int)G, 1);
+        const size_t need =
+            chunk_bytes(N, TL, v[i]->dev->num_layers, j0.dev->Nx, j0.dev->Ny, hx, hy,
+                        !j0.dev->grids_on_device, (int)G, std::max(1, std::min(TL, kMinLayerBatch)));
         if (need <= c.budget || G == 1) break;
         G = (G + 1) / 2;
       }

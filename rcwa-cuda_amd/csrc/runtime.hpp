@@ -55,6 +55,11 @@ struct LayerWork {
 };
 
 size_t layer_work_bytes(int N, int Bl);
+// Layer sub-batch that chunk sizing reserves room for: the eigensolver runs
+// one workgroup per matrix, so a launch needs about one matrix per CU (256 on
+// MI355X) to fill the chip.  Admission keeps at least this much HBM for the
+// layer stage before it adds structures to a chunk.
+constexpr int kMinLayerBatch = 256;
 size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool host_grids,
                    int G, int Bl);
 

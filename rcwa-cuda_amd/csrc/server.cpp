@@ -10,7 +10,9 @@
 //   1. take the oldest pending request; its size bucket is (harmonics, grid);
 //   2. admit further pending requests of that bucket, oldest first, while the
 //      chunk's HBM footprint (chunk_bytes: layers, S-matrix slots, fold
-//      temporaries, workspaces) fits the worker's budget, up to max_batch;
+//      temporaries, workspaces for a layer sub-batch of kMinLayerBatch, so
+//      that the layer launches still fill the chip) fits the worker's
+//      budget, up to max_batch;
 //   3. run them as one batched launch sequence (layers of all admitted
 //      requests in one layer batch; the fold batched over requests, each
 //      request leaving the fold after its own last layer);
@@ -99,9 +101,10 @@ struct rcwa_server {
             continue;
           }
           const int L = r->dev.num_layers;
-          const size_t need =
-              chunk_bytes(r->N, TL + L, std::max(Lmax, L), r->dev.Nx, r->dev.Ny,
-                          r->par.Nx_harmonics, r->par.Ny_harmonics, true, (int)batch.size() + 1, 1);
+          const size_t need = chunk_bytes(
+              r->N, TL + L, std::max(Lmax, L), r->dev.Nx, r->dev.Ny, r->par.Nx_harmonics,
+              r->par.Ny_harmonics, true, (int)batch.size() + 1,
+              std::max(1, std::min(TL + L, kMinLayerBatch)));
           if (!batch.empty() && need > c.budget) break;  // HBM admission
           TL += L;
           Lmax = std::max(Lmax, L);
