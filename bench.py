@@ -142,6 +142,30 @@ def serve_bench(args):
             "max_dR_mix_vs_alone": dmax, "clock_start": c0}
 
 
+def c5_bench():
+    """The large single structure: 41x41 harmonics (n = 3362), 50 layers on
+    1024^2 grids, one solve from host grids.  Reports the wall time, the
+    stage times and the algorithmic rate."""
+    import rcwa_amd as G
+    import workloads as W
+    p = W.c5_large()
+    N = 41 * 41
+    ctx = G.Context(0)
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    r = ctx.solve([p])[0]
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    flops = (2384.0 * 50 + 1621.0) * N ** 3
+    return {"workload": "41x41 harmonics (N=1681, 3362x3362 complex128), 50 layers, 1024x1024 "
+                        "grids, lambda 400 nm, theta 30 deg, TE; host grids",
+            "seconds": wall, "tflops_algorithmic": flops / wall / 1e12,
+            "R_tot": float(r[2][0][0]), "T_tot": float(r[2][0][1]),
+            "stage_ms": {k: st[k] for k in ("ms_convmat", "ms_layers", "ms_eig", "ms_eig_hess",
+                                            "ms_eig_schur", "ms_eig_vec", "ms_fold")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +174,8 @@ def main():
     ap.add_argument("--lams", type=int, default=48, help="wavelengths per step")
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c5", action="store_true",
+                    help="also time the 41x41-harmonic, 50-layer single structure")
     ap.add_argument("--serve-percall", type=int, default=24,
                     help="requests of the stream timed one per call")
     ap.add_argument("--serve", type=int, default=0,
@@ -273,6 +299,9 @@ def main():
 
     if args.serve:
         out["serving"] = serve_bench(args)
+
+    if args.c5:
+        out["c5"] = c5_bench()
 
     if cpu_procs:
         t = {}

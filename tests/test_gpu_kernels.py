@@ -136,6 +136,22 @@ def test_eig_workload_size(ctx):
         assert _match_eigs(wi, np.linalg.eigvals(a)) < 1e-9 * nrm
 
 
+def test_eig_large_sampled_residual(ctx):
+    """n = 2100 (the one-workgroup sweep kernel and the log apply at large n,
+    register Gauss-Jordan with two rows per thread in the caller): residual
+    on sampled eigenpairs, eigenvalues against LAPACK on a smaller copy
+    would be too slow here, so their sum is checked against the trace."""
+    n = 2100
+    rng = np.random.default_rng(2100)
+    a = rnd(rng, n, n)
+    w, V = ctx.eig([a])
+    w, V = w[0], V[0]
+    cols = np.arange(0, n, 97)
+    res = np.abs(a @ V[:, cols] - V[:, cols] * w[None, cols]).max() / np.abs(a).max()
+    assert res < 1e-12 * n, res
+    assert abs(w.sum() - np.trace(a)) < 1e-9 * np.abs(a).max() * n
+
+
 def test_eig_degenerate_diagonal(ctx):
     # a uniform layer gives a diagonal Omega^2 with every eigenvalue twice
     d = np.repeat(np.arange(1, 6) - 3.5 + 0.1j, 2)
