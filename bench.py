@@ -228,8 +228,12 @@ def main():
         grids = [(er_devs[chunk * args.lams + i], ur_dev) for i in range(len(sel))]
         return ctx.solve(sel, precision, device_grids=grids, grid_shape=(512, 512))
 
+    def progress(msg):
+        print("[bench] " + msg, file=sys.stderr, flush=True)
+
     for w in range(args.warmup):
         run(w)
+        progress("warm-up step %d done" % (w + 1))
     hip.sync()
     ctx.reset_stats()
     ctx.set_timing(True)
@@ -240,6 +244,7 @@ def main():
         t0 = time.perf_counter()
         results.append(run(args.warmup + k))
         step_ms.append((time.perf_counter() - t0) * 1e3)
+        progress("timed step %d done (%.0f ms)" % (k + 1, step_ms[-1]))
     hip.sync()
     elapsed = time.perf_counter() - t_start
     st = ctx.stats()
@@ -290,6 +295,7 @@ def main():
         t0 = time.perf_counter()
         for k in range(args.steps):
             sres.append(run(args.warmup + k, G.RCWA_SPLIT))
+            progress("split-precision step %d done" % (k + 1))
         hip.sync()
         selapsed = time.perf_counter() - t0
         dR = max(np.abs(a[0] - b[0]).max() for ra, rb in zip(results, sres) for a, b in zip(ra, rb))
@@ -298,9 +304,11 @@ def main():
                         "max_dR_vs_fp64": float(dR), "max_dT_vs_fp64": float(dT)}
 
     if args.serve:
+        progress("serving stream of %d requests" % args.serve)
         out["serving"] = serve_bench(args)
 
     if args.c5:
+        progress("41x41-harmonic single structure")
         out["c5"] = c5_bench()
 
     if cpu_procs:
