@@ -125,3 +125,17 @@ def test_solve_batch_mixed_layer_counts(ctx):
     out = ctx.solve(probs)
     for p, g in zip(probs, out):
         assert max(_cmp(g, O.solve(p))) < TOL
+
+
+def test_line_grating_sweep_vs_oracle(ctx):
+    """The benchmark's structure at a testable size: the trapezoid line
+    grating of 20 slices over 5 dispersive materials, theta = 65 deg, TE and
+    TM, 7x7 harmonics on 64x64 grids, at wavelengths across the 250-800 nm
+    sweep (including the substrate cutoff region near 289 nm), solved as one
+    batch and compared per order with the oracle."""
+    probs = W.c2_sweep(indices=[0, 14, 70, 199], grid=64, h=3)
+    out = ctx.solve(probs)
+    for p, gpu in zip(probs, out):
+        ref = O.solve(p)
+        dR, dT, dtot = _cmp(gpu, ref)
+        assert max(dR, dT, dtot) < TOL, (p.lam0, dR, dT, dtot)
