@@ -151,6 +151,27 @@ __device__ __forceinline__ void givens(zc f, zc g, double& c, zc& s, zc& r) {
   r = zscale(ph, nr);
 }
 
+// The rotation of givens() from one reciprocal square root t = (|f|^2 h)^-1/2,
+// h = |f|^2 + |g|^2: c = |f|^2 t, s = f conj(g) t, r = f h t.  Used when
+// 2^-500 <= |f|^2 and h <= 2^500 (every intermediate then stays normal);
+// otherwise givens().  v_rsq_f64 is refined by two Newton steps.
+__device__ __forceinline__ void givens_rsq(zc f, zc g, double& c, zc& s, zc& r) {
+  const double f2 = f.x * f.x + f.y * f.y, g2 = g.x * g.x + g.y * g.y, h = f2 + g2;
+  if (f2 >= 0x1p-500 && h <= 0x1p500) {
+    const double p = f2 * h;
+    double t = __builtin_amdgcn_rsq(p);
+    double e = fma(-p * t, t, 1.0);
+    t = fma(0.5 * t, e, t);
+    e = fma(-p * t, t, 1.0);
+    t = fma(0.5 * t, e, t);
+    c = f2 * t;
+    s = zscale(zmul(f, zconj(g)), t);
+    r = zscale(f, h * t);
+  } else {
+    givens(f, g, c, s, r);
+  }
+}
+
 __device__ zc wilkinson(zc a, zc b, zc c, zc d) {
   const zc x = zscale(zsub(a, d), 0.5);
   const zc bc = zmul(b, c);
@@ -942,6 +963,419 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
   }
 }
 
+// Raw buffer access to one matrix: offsets at or past the end read zero and
+// drop stores, so masked-off lanes need no branch (and the loads of a batch
+// stay in flight together).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t matrix_rsrc(const zc* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<zc*>(p), 0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ zc buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return zmake(__builtin_bit_cast(double, ((unsigned long long)v.y << 32) | v.x),
+               __builtin_bit_cast(double, ((unsigned long long)v.w << 32) | v.z));
+}
+
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, zc z) {
+  const unsigned long long a = __builtin_bit_cast(unsigned long long, z.x);
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, z.y);
+  u32x4 v;
+  v.x = (unsigned)a;
+  v.y = (unsigned)(a >> 32);
+  v.z = (unsigned)b;
+  v.w = (unsigned)(b >> 32);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
+}
+
+// Window rotations applied to rows x[Q0 .. Q0 + nr] of one column held in
+// registers (row rotations: T <- G^H T).
+template <int R, int Q0>
+__device__ __forceinline__ void rotate_reg_rows(zc* x, int nr, const double* rc, const zc* rs) {
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (q < nr) {
+      const double c = rc[q];
+      const zc sr = rs[q], sc = zconj(sr);
+      const zc a = x[Q0 + q], b = x[Q0 + q + 1];
+      x[Q0 + q] = zadd(zscale(a, c), zmul(sr, b));
+      x[Q0 + q + 1] = zsub(zscale(b, c), zmul(sc, a));
+    }
+}
+
+// One strip segment: a column's rows (colform false, row rotations) or a
+// row's columns (colform true, column rotations use the conjugate sine).
+// Rotations past nr are replaced by the identity (entries there are zero or
+// finite), so the code has no branches.
+template <int R>
+__device__ __forceinline__ void rotate_segment(zc* x, int nr, bool colform, const double* rc,
+                                               const zc* rs) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const bool on = q < nr;
+    const double c = on ? rc[q] : 1.0;
+    zc s = on ? rs[q] : zmake(0, 0);
+    if (colform) s.y = -s.y;
+    const zc a = x[q], b = x[q + 1];
+    x[q] = zadd(zscale(a, c), zmul(s, b));
+    x[q + 1] = zsub(zscale(b, c), zmul(zconj(s), a));
+  }
+}
+
+// One workgroup per matrix, same iteration and log as schur_defer_kernel,
+// with wave 0's memory traffic taken off its critical path.  Windows: window
+// x holds rotations k0(x) .. k1(x)-1 (rows k0(x) .. k1(x)), its band is
+// rows/columns b0(x) .. ce(x) (ce clamped to iu); J(x) = (ce(x-1), ce(x)] are
+// the columns band x adds.  In step w wave 0
+//   - issues the loads for window w+1: rows k0(w-1) .. k1(w) of its lane's
+//     column of J(w+1), and band w+1's entries that no earlier window
+//     touches (rows and columns right/below its overlap with band w);
+//   - stores band w-1 and logs window w-1's rotations (their first readers
+//     in memory come after the next barrier);
+//   - applies window w-1's rotations to its column of J(w) (loaded one step
+//     earlier), stores the rows above band w and puts the two rows inside it
+//     into LDS; assembles the rest of band w from band w-1 (LDS), the
+//     prefetched entries and zeros below the subdiagonal;
+//   - chases window w through band w;
+//   - applies window w-1's rotations to the top rows of the column loaded
+//     for J(w+1).
+// The other waves apply window w-1's rotations to the columns right of
+// ce(w+1) and the rows above b0(w-1), MI segments per thread in flight.
+// Every region has one writer per step: wave 0 owns the bands and J(w),
+// J(w+1); the strips own the rest.
+template <int R, int NT, int MI>
+__global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm, int* info,
+                                                        int maxit_per_row, QrLog log) {
+  constexpr int LDB = R + 3;
+  constexpr int PF = ((R + 1) * (R + 1) + 63) / 64;
+  constexpr int SW = NT / 64 - 1, RP = R + 1;
+  constexpr int JR = 2 * R + 1, JL = (R * JR + 63) / 64;  // J column rows; loads per lane
+  constexpr int JS = 2 * R - 1, JSL = (R * JS + 63) / 64;  // J rows stored per step
+  constexpr int BL = (LDB * LDB + 63) / 64;
+  __shared__ zc band[2][LDB * LDB];
+  __shared__ zc stile[SW * MI * 64 * RP];  // each strip wave's transpose tiles
+  __shared__ zc jcol[R * JR];              // wave 0: column l of J(w) at l * JR, 2R+1 rows
+  __shared__ double rc[2][R];  // rotations of the current and the previous window
+  __shared__ zc rs[2][R];
+  __shared__ int s_il;
+  __shared__ zc s_mu;
+  const int b = blockIdx.x;
+  zc* T = Tm.p + (long long)b * Tm.stride;
+  const QrLogM L = qr_log_of(log, b);
+  const long long ld = Tm.ld;
+  const unsigned tbytes = (unsigned)(ld * n * 16);  // below 2^31 (checked by the host)
+  const __amdgpu_buffer_rsrc_t trs = matrix_rsrc(T, tbytes);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double eps = DBL_EPSILON;
+  const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
+  int* st = log.state + 4 * b;
+  int iu = st[0], iter = st[1], nsw = 0, used = 0;
+  long long total = st[2];
+  if (st[3] != QR_RUNNING) {
+    if (tid == 0) log.nsw[b] = 0;
+    return;
+  }
+  bool failed = false, stopped = false;
+  for (;;) {
+    if (iu <= 0 || failed) break;
+    if (tid == 0) s_il = 0;
+    __syncthreads();
+    for (int k = iu - tid; k >= 1; k -= NT) {
+      const double sub = zabs1(T[k + (k - 1) * ld]);
+      const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
+      if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
+    }
+    __syncthreads();
+    const int il = __builtin_amdgcn_readfirstlane(s_il);
+    if (il == iu) {
+      if (tid == 0) T[iu + (iu - 1) * ld] = zmake(0, 0);
+      --iu;
+      iter = 0;
+      __syncthreads();
+      continue;
+    }
+    if (nsw == log.scap || (long long)used + (iu - il) > log.limit) {
+      stopped = true;
+      break;
+    }
+    ++iter;
+    ++total;
+    if (total > maxit) {
+      if (tid == 0 && info) info[b] = 1;
+      failed = true;
+      continue;
+    }
+    if (tid == 0) {
+      if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
+      zc mu;
+      if (iter == 10)
+        mu = zadd(T[il + il * ld], zmake(0.75 * fabs(T[(il + 1) + il * ld].x), 0));
+      else if (iter == 20)
+        mu = zadd(T[iu + iu * ld], zmake(0.75 * fabs(T[iu + (iu - 1) * ld].x), 0));
+      else
+        mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
+                       T[iu + (iu - 1) * ld], T[iu + iu * ld]);
+      s_mu = mu;
+      L.il[nsw] = il;
+      L.iu[nsw] = iu;
+      L.off[nsw] = used;
+    }
+    __syncthreads();
+    const int nw = (iu - il + R - 1) / R;
+    auto wk0 = [&](int x) { return il + x * R; };
+    auto wk1 = [&](int x) { return min(il + x * R + R, iu); };
+    auto wce = [&](int x) { return x < nw ? min(il + x * R + R + 1, iu) : iu; };
+    auto wb0 = [&](int x) { return x > 0 ? il + x * R - 1 : il; };
+    for (int w = 0; w <= nw; ++w) {
+      const bool cur = w < nw, prev = w > 0;
+      const int k0 = wk0(w), k1 = cur ? wk1(w) : k0, nr = k1 - k0;
+      const int b0 = wb0(w), ce = wce(w), nb = ce - b0 + 1;
+      const int pk0 = prev ? wk0(w - 1) : 0, pnr = prev ? wk1(w - 1) - pk0 : 0;
+      const int pb0 = prev ? wb0(w - 1) : 0, pce = prev ? wce(w - 1) : 0;
+      const double* prc = rc[(w + 1) & 1];  // window w-1's rotations
+      const zc* prs = rs[(w + 1) & 1];
+      zc* cb = band[w & 1];
+      const zc* pbnd = band[(w + 1) & 1];
+      if (wave == 0) {
+        // loads for window w+1: rows pk0 .. pk0+2R of J(w+1) (in step 0 rows
+        // k0 .. k0+R, at R .. 2R), element e = column e / JR, row e % JR
+        const int ce1 = wce(w + 1);
+        const int nj1 = cur ? ce1 - ce : 0;
+        const int jr0 = prev ? pk0 : k0 - R;  // row of element q = 0
+        zc yv[JL];
+#pragma unroll
+        for (int i = 0; i < JL; ++i) {
+          const int e = lane + 64 * i, col = e / JR, q = e % JR;
+          const bool ok = col < nj1 && (prev || q >= R);
+          yv[i] = buf_load(
+              trs, ok ? (unsigned)((jr0 + q) + (long long)(ce + 1 + col) * ld) * 16u : tbytes);
+        }
+        zc pf[PF];
+        const int m1 = ce1 - k1;  // band w+1's fresh block: rows/columns k1+1 .. ce1
+        if (w + 1 < nw) {
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            const int idx = lane + 64 * p;
+            const int i = idx % m1, j = idx / m1;
+            pf[p] = buf_load(trs, idx < m1 * m1 && idx != 0 && i <= j + 1
+                                      ? (unsigned)((k1 + 1 + i) + (long long)(k1 + 1 + j) * ld) * 16u
+                                      : tbytes);
+          }
+        }
+        if (prev) {
+          // band w-1 (upper Hessenberg part) and window w-1's rotations
+          const int pnb = pce - pb0 + 1;
+#pragma unroll
+          for (int p = 0; p < BL; ++p) {
+            const int idx = lane + 64 * p, i = idx % pnb, j = idx / pnb;
+            const bool ok = idx < pnb * pnb && i <= j + 1;
+            buf_store(trs, ok ? (unsigned)((pb0 + i) + (long long)(pb0 + j) * ld) * 16u : tbytes,
+                      pbnd[ok ? i + j * LDB : 0]);
+          }
+          if (lane < pnr) {
+            L.c[used + (pk0 - il) + lane] = prc[lane];
+            L.s[used + (pk0 - il) + lane] = prs[lane];
+          }
+        }
+        if (cur && prev) {
+          // J(w): column l in jcol holds rows pk0-R .. pk0+R; window w-1's
+          // rotations on rows pk0 .. pk0+R, rows above band w to memory, the
+          // two rows inside it (k0-1, k0) into LDS
+          const int nj = ce - pce;
+          if (lane < nj) {
+            zc xr[R + 1];
+#pragma unroll
+            for (int q = 0; q <= R; ++q) xr[q] = jcol[lane * JR + R + q];
+            rotate_reg_rows<R, 0>(xr, pnr, prc, prs);
+#pragma unroll
+            for (int q = 0; q < R - 1; ++q) jcol[lane * JR + R + q] = xr[q];
+            cb[0 + (pce + 1 + lane - b0) * LDB] = xr[R - 1];
+            cb[1 + (pce + 1 + lane - b0) * LDB] = xr[R];
+          }
+#pragma unroll
+          for (int i = 0; i < JSL; ++i) {
+            const int e = lane + 64 * i, col = e / JS, q = e % JS;
+            const bool ok = col < nj && (w >= 2 || q >= R);
+            buf_store(trs, ok ? (unsigned)((pk0 - R + q) + (long long)(pce + 1 + col) * ld) * 16u
+                              : tbytes,
+                      jcol[ok ? col * JR + q : 0]);
+          }
+          // overlap with band w-1: rows/columns k0-1 .. k0+1
+          if (lane < 9) {
+            const int i = lane % 3, j = lane / 3, o = b0 - pb0;
+            cb[i + j * LDB] = pbnd[(o + i) + (o + j) * LDB];
+          }
+          // rows k0+2 .. ce of columns k0-1, k0
+          const int mz = ce - k0 - 1;
+          for (int idx = lane; idx < 2 * mz; idx += 64)
+            cb[(3 + idx % mz) + (idx / mz) * LDB] = zmake(0, 0);
+        } else if (cur) {
+          for (int idx = lane; idx < nb * nb; idx += 64) {
+            const int i = idx % nb, j = idx / nb;
+            cb[i + j * LDB] =
+                i <= j + 1 ? T[(b0 + i) + (long long)(b0 + j) * ld] : zmake(0, 0);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (cur) {
+          double* crc = rc[w & 1];
+          zc* crs = rs[w & 1];
+          for (int k = k0; k < k1; ++k) {
+            // every lane computes the same rotation (no broadcast)
+            double c;
+            zc sr, r, f, g;
+            if (k == il) {
+              f = zsub(cb[(k - b0) + (k - b0) * LDB], s_mu);
+              g = cb[(k + 1 - b0) + (k - b0) * LDB];
+            } else {
+              f = cb[(k - b0) + (k - 1 - b0) * LDB];
+              g = cb[(k + 1 - b0) + (k - 1 - b0) * LDB];
+            }
+            givens_rsq(f, g, c, sr, r);
+            if (lane == 0) {
+              if (k > il) {
+                cb[(k - b0) + (k - 1 - b0) * LDB] = r;
+                cb[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
+              }
+              crc[k - k0] = c;
+              crs[k - k0] = sr;
+            }
+            const zc sc = zconj(sr);
+            for (int j = k + lane; j <= ce; j += 64) {
+              zc* p0 = &cb[(k - b0) + (j - b0) * LDB];
+              const zc x = p0[0], y = p0[1];
+              p0[0] = zadd(zscale(x, c), zmul(sr, y));
+              p0[1] = zsub(zscale(y, c), zmul(sc, x));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            const int ie = min(k + 2, iu);
+            for (int i = b0 + lane; i <= ie; i += 64) {
+              zc* q0 = &cb[(i - b0) + (k - b0) * LDB];
+              const zc x = q0[0], y = q0[LDB];
+              q0[0] = zadd(zscale(x, c), zmul(sc, y));
+              q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+        if (nj1 > 0) {
+          // J(w+1) into jcol; window w-1's rotations on its rows pk0 .. pk0+R
+#pragma unroll
+          for (int i = 0; i < JL; ++i)
+            if (lane + 64 * i < R * JR) jcol[lane + 64 * i] = yv[i];
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          if (prev && lane < nj1) {
+            zc xr[R + 1];
+#pragma unroll
+            for (int q = 0; q <= R; ++q) xr[q] = jcol[lane * JR + q];
+            rotate_reg_rows<R, 0>(xr, pnr, prc, prs);
+#pragma unroll
+            for (int q = 0; q <= R; ++q) jcol[lane * JR + q] = xr[q];
+          }
+        }
+        if (w + 1 < nw) {
+          // band w+1's rows/columns k1+1 .. ce1, less the corner (band w-1's
+          // buffer, read for the last time above)
+          zc* nbnd = band[(w + 1) & 1];
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            const int idx = lane + 64 * p;
+            if (idx < m1 * m1 && idx != 0) nbnd[(2 + idx % m1) + (2 + idx / m1) * LDB] = pf[p];
+          }
+        }
+      } else if (prev) {
+        // chunks of 64 columns of the right strip (loaded row-fastest, turned
+        // into one column per lane through this wave's LDS tile) and of 64
+        // rows of the top strip (one row per lane), MI chunks in flight
+        const int sw = wave - 1;
+        const int r0 = wce(w + 1) + 1;  // right strip: columns r0 .. iu
+        const int nright = max(iu - r0 + 1, 0);
+        const int ntop = max(pb0 - il, 0);  // top strip: rows il .. pb0-1
+        const int nrc = (nright + 63) / 64, nch = nrc + (ntop + 63) / 64;
+        zc* tile = stile + sw * (MI * 64 * RP);
+        for (int c0 = sw; c0 < nch; c0 += SW * MI) {
+          zc v[MI][RP];
+          unsigned off[MI][RP];
+#pragma unroll
+          for (int mi = 0; mi < MI; ++mi) {
+            const int ch = c0 + mi * SW;
+            if (ch < nrc) {
+              const int cb = r0 + ch * 64, ncol = min(64, iu - cb + 1);
+#pragma unroll
+              for (int i = 0; i < RP; ++i) {
+                const int e = lane + 64 * i, col = e / RP, row = e % RP;
+                off[mi][i] = col < ncol && row <= pnr
+                                 ? (unsigned)((pk0 + row) + (long long)(cb + col) * ld) * 16u
+                                 : tbytes;
+              }
+            } else {
+              const int row = il + (ch - nrc) * 64 + lane;
+              const bool okr = ch < nch && row < pb0;
+#pragma unroll
+              for (int q = 0; q < RP; ++q)
+                off[mi][q] = okr && q <= pnr
+                                 ? (unsigned)(row + (long long)(pk0 + q) * ld) * 16u
+                                 : tbytes;
+            }
+#pragma unroll
+            for (int i = 0; i < RP; ++i) v[mi][i] = buf_load(trs, off[mi][i]);
+          }
+#pragma unroll
+          for (int mi = 0; mi < MI; ++mi)
+            if (c0 + mi * SW < nrc) {
+#pragma unroll
+              for (int i = 0; i < RP; ++i) tile[mi * 64 * RP + lane + 64 * i] = v[mi][i];
+            }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int mi = 0; mi < MI; ++mi) {
+            const bool right = c0 + mi * SW < nrc;
+            if (right) {
+#pragma unroll
+              for (int q = 0; q < RP; ++q) v[mi][q] = tile[mi * 64 * RP + lane * RP + q];
+            }
+            rotate_segment<R>(v[mi], pnr, !right, prc, prs);
+            if (right) {
+#pragma unroll
+              for (int q = 0; q < RP; ++q) tile[mi * 64 * RP + lane * RP + q] = v[mi][q];
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+#pragma unroll
+          for (int mi = 0; mi < MI; ++mi) {
+            if (c0 + mi * SW < nrc) {
+#pragma unroll
+              for (int i = 0; i < RP; ++i) v[mi][i] = tile[mi * 64 * RP + lane + 64 * i];
+            }
+#pragma unroll
+            for (int i = 0; i < RP; ++i) buf_store(trs, off[mi][i], v[mi][i]);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __syncthreads();
+    }
+    used += iu - il;
+    ++nsw;
+  }
+  if (tid == 0) {
+    log.nsw[b] = nsw;
+    st[0] = iu;
+    st[1] = iter;
+    st[2] = (int)total;
+    st[3] = stopped ? QR_RUNNING : failed ? QR_FAILED : QR_DONE;
+    if (stopped) atomicAdd(log.pending, 1);
+  }
+}
+
 __global__ void qr_state_init_kernel(int n, int batch, int* state, int* pending) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0) *pending = 0;
@@ -1020,12 +1454,13 @@ size_t schur_ws_bytes(int n, int batch) {
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                   int maxit_per_row, void* ws, long long log_cap) {
   if (n <= 0 || batch <= 0) return 0;
-  // RCWA_QR_SWEEP=direct selects the unwindowed sweep, =window the windowed
-  // sweep that updates all of T and Z at once (same arithmetic, same result).
-  static const char mode = [] {
-    const char* e = getenv("RCWA_QR_SWEEP");
-    return e ? e[0] : 0;
-  }();
+  // RCWA_QR_SWEEP (read on every call) selects another form of the same
+  // iteration: =direct the unwindowed sweep, =window the windowed sweep that
+  // updates all of T and Z at once, =oneshot the logged sweep without the
+  // pipelined wave 0 (also used for matrices of 2^31 bytes or more, which
+  // the pipelined kernel's 32-bit buffer offsets cannot address).
+  const char* mode_env = getenv("RCWA_QR_SWEEP");
+  const char mode = mode_env ? mode_env[0] : 0;
   if (mode == 'd') return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
   if (ws && mode != 'w') {
     auto align = [](uintptr_t p) { return (p + 255) & ~(uintptr_t)255; };
@@ -1053,8 +1488,12 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
     // round makes progress on every pending matrix (at least one sweep), and
     // the sweep limit bounds the sweeps, so the loop ends.
     for (;;) {
-      hipLaunchKernelGGL((schur_defer_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z,
-                         info, maxit_per_row, log);
+      if (mode == 'o' || (long long)T.ld * n * 16 >= (1LL << 31))
+        hipLaunchKernelGGL((schur_defer_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z,
+                           info, maxit_per_row, log);
+      else
+        hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 1>), dim3(batch), dim3(256), 0, s, n, T,
+                           Z, info, maxit_per_row, log);
       if ((e = (int)hipGetLastError())) return e;
       hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
                          dim3((2 * n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n,

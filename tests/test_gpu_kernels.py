@@ -123,6 +123,31 @@ def test_eig_small_rotation_log(cap):
         assert _match_eigs(wi, np.linalg.eigvals(m)) < 1e-10 * nrm
 
 
+@pytest.mark.parametrize("mode", ["", "oneshot", "window", "direct"])
+def test_eig_sweep_forms(ctx, monkeypatch, mode):
+    """Every form of the QR sweep (the default pipelined one, the one-shot
+    logged sweep used for matrices of 2^31 bytes or more, the windowed and
+    the unwindowed full-update sweeps) on a matrix that splits in the middle,
+    with a rotation log small enough to fill and resume."""
+    if mode:
+        monkeypatch.setenv("RCWA_QR_SWEEP", mode)
+    n = 170
+    rng = np.random.default_rng(17)
+    a = rnd(rng, n, n)
+    a[110:, :110] = 0
+    c = G.Context(0)
+    try:
+        c.set_qr_log(2000)
+        w, V = c.eig([a, a.T.copy()])
+    finally:
+        c.close()
+    for m, wi, vi in zip([a, a.T], w, V):
+        nrm = np.abs(m).max()
+        res = np.abs(m @ vi - vi * wi[None, :]).max() / nrm
+        assert res < 1e-12 * n, res
+        assert _match_eigs(wi, np.linalg.eigvals(m)) < 1e-10 * nrm
+
+
 def test_eig_workload_size(ctx):
     """n = 882: the Omega^2 size of 21 x 21 harmonics."""
     n = 882
