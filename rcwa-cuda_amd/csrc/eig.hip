@@ -639,18 +639,29 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
         (c >= cfirst && c <= min(T0, hi)) ? base[(long long)c * step] : zmake(0, 0);
   }
   w[1] = zmake(0, 0);  // the slot of column T0 + 1, which the first step loads
+  // A block's incoming columns tb+1 .. tb+W are loaded one block ahead, so
+  // the loads run under the previous block's rotations.  No rotation reaches
+  // column cn before step cn - 1 and no earlier step stores it, so the early
+  // load reads its final value; its slot is free when step cn - 1 fills it
+  // (the slot's previous column, cn - W, was stored at step cn - 2).  A
+  // column the preload already holds is reloaded unchanged.
+  // Loads are unconditional (no branch, so the compiler keeps the next
+  // block's loads in flight): a column outside lo .. hi is read at the
+  // nearest end of the range instead; such a slot only ever meets identity
+  // rotations (1 * x + 0 * y with finite y) and is never stored.
+  zc pre[W];
+#pragma unroll
+  for (int u = 0; u < W; ++u) {
+    const int cn = min(max(T0 + u + 1, lo), hi);
+    pre[u] = base[(long long)cn * step];
+  }
   int tb = T0;
   for (; tb <= T1; tb += W) {
-    // The block's incoming columns tb+1 .. tb+W, loaded together.  The slot of
-    // column cn is free when step cn - 1 fills it: its previous column,
-    // cn - W, was stored at step cn - 2.  A column the preload already holds
-    // is reloaded unchanged (no rotation has reached it yet).
-    zc pre[W];
+    zc nxt[W];
 #pragma unroll
     for (int u = 0; u < W; ++u) {
-      const int cn = tb + u + 1;
-      pre[u] = zmake(0, 0);
-      if (cn >= lo && cn <= hi && cn >= first) pre[u] = base[(long long)cn * step];
+      const int cn = min(max(tb + W + u + 1, lo), hi);
+      nxt[u] = base[(long long)cn * step];
     }
 #pragma unroll
     for (int u = 0; u < W; ++u) {
@@ -677,6 +688,8 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
       if (done >= lo && done <= hi && done >= first)
         base[(long long)done * step] = w[(u - 2 * (G - 1)) & (W - 1)];
     }
+#pragma unroll
+    for (int u = 0; u < W; ++u) pre[u] = nxt[u];
   }
   // Still in flight after the last executed step tend = tb - 1 (tb is a
   // multiple of W): columns tb - 2(G-1) .. tb, again in static slots.
@@ -989,83 +1002,95 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off
   __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
 }
 
-// Window rotations applied to rows x[Q0 .. Q0 + nr] of one column held in
-// registers (row rotations: T <- G^H T).
-template <int R, int Q0>
-__device__ __forceinline__ void rotate_reg_rows(zc* x, int nr, const double* rc, const zc* rs) {
+// The rotations of one window applied in order to RS = R + GAP (NC - 1) + 1
+// consecutive entries held in registers: a column's rows (colform false, row
+// rotations) or a row's columns (colform true, column rotations, which use
+// the conjugate sine).  Entry NC t + c is chain c's rotation at step t and
+// acts on entries GAP (NC - 1 - c) + t and the next; entries with no rotation
+// hold the identity.
+template <int R, int NC, int GAP>
+__device__ __forceinline__ void rotate_window(zc* x, bool colform, const double* rc,
+                                              const zc* rs) {
 #pragma unroll
-  for (int q = 0; q < R; ++q)
-    if (q < nr) {
-      const double c = rc[q];
-      const zc sr = rs[q], sc = zconj(sr);
-      const zc a = x[Q0 + q], b = x[Q0 + q + 1];
-      x[Q0 + q] = zadd(zscale(a, c), zmul(sr, b));
-      x[Q0 + q + 1] = zsub(zscale(b, c), zmul(sc, a));
+  for (int t = 0; t < R; ++t)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int e = NC * t + c, o = GAP * (NC - 1 - c) + t;
+      const double cs = rc[e];
+      zc s = rs[e];
+      if (colform) s.y = -s.y;
+      const zc a = x[o], b = x[o + 1];
+      x[o] = zadd(zscale(a, cs), zmul(s, b));
+      x[o + 1] = zsub(zscale(b, cs), zmul(zconj(s), a));
     }
 }
 
-// One strip segment: a column's rows (colform false, row rotations) or a
-// row's columns (colform true, column rotations use the conjugate sine).
-// Rotations past nr are replaced by the identity (entries there are zero or
-// finite), so the code has no branches.
-template <int R>
-__device__ __forceinline__ void rotate_segment(zc* x, int nr, bool colform, const double* rc,
-                                               const zc* rs) {
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const bool on = q < nr;
-    const double c = on ? rc[q] : 1.0;
-    zc s = on ? rs[q] : zmake(0, 0);
-    if (colform) s.y = -s.y;
-    const zc a = x[q], b = x[q + 1];
-    x[q] = zadd(zscale(a, c), zmul(s, b));
-    x[q + 1] = zsub(zscale(b, c), zmul(zconj(s), a));
-  }
-}
-
-// One workgroup per matrix, same iteration and log as schur_defer_kernel,
-// with wave 0's memory traffic taken off its critical path.  Windows: window
-// x holds rotations k0(x) .. k1(x)-1 (rows k0(x) .. k1(x)), its band is
-// rows/columns b0(x) .. ce(x) (ce clamped to iu); J(x) = (ce(x-1), ce(x)] are
-// the columns band x adds.  In step w wave 0
-//   - issues the loads for window w+1: rows k0(w-1) .. k1(w) of its lane's
-//     column of J(w+1), and band w+1's entries that no earlier window
-//     touches (rows and columns right/below its overlap with band w);
-//   - stores band w-1 and logs window w-1's rotations (their first readers
-//     in memory come after the next barrier);
-//   - applies window w-1's rotations to its column of J(w) (loaded one step
-//     earlier), stores the rows above band w and puts the two rows inside it
-//     into LDS; assembles the rest of band w from band w-1 (LDS), the
-//     prefetched entries and zeros below the subdiagonal;
-//   - chases window w through band w;
-//   - applies window w-1's rotations to the top rows of the column loaded
-//     for J(w+1).
+// One workgroup per matrix: the iteration of schur_defer_kernel (same shifts,
+// deflation and rotation log), with the bulge chase pipelined and, for
+// NC = 2, two sweeps chased together.
+//
+// Two chains.  A pass over the active block il..iu runs NC sweeps with the
+// same Wilkinson shift: chain 0, and chain 1 (NC = 2, blocks of at least 2R
+// rows, not on the sweeps that take an exceptional shift) GAP = 2 positions
+// behind.  At step t chain 0
+// rotates at position k and chain 1 at k - GAP: their row rotations act on
+// different rows, their column rotations on different columns, and a row
+// rotation of one commutes with a column rotation of the other, so the pass
+// equals the two sweeps one after the other -- which is how the log records
+// them, and how the log is applied.
+//
+// Windows.  Window x holds steps il + xR .. il + xR + R - 1 of chain 0
+// (k0(x) = il + xR); its rotations span rows k0 - GP .. k0 + R (GP = GAP
+// (NC - 1)); its band is rows/columns b0(x) = k0 - GP - 1 (il for x = 0) ..
+// ce(x) = min(k0 + R + 1, iu).  J(x) = (ce(x-1), ce(x)] are the columns band
+// x adds.  In step w wave 0
+//   - issues the loads for window w+1: rows k0(w) - R - GP .. k0(w) + R of its
+//     column of J(w+1) (element e: column e / JR, row e % JR), and band w+1's
+//     fresh block (rows/columns ce(w) .. ce(w+1), less the corner);
+//   - stores band w-1 and logs window w-1's rotations (their first readers in
+//     memory come after the next barrier);
+//   - applies window w-1's rotations to the columns of J(w) (loaded a step
+//     earlier and held in LDS), stores the rows above band w and puts the
+//     GP + 2 rows inside it into LDS; copies the overlap from band w-1 and
+//     zeros the entries below the subdiagonal that no other source fills;
+//   - chases window w through band w, chain c in lanes c * 64/NC ..;
+//   - applies window w-1's rotations to the top rows of J(w+1) and writes
+//     band w+1's prefetched block into LDS.
 // The other waves apply window w-1's rotations to the columns right of
-// ce(w+1) and the rows above b0(w-1), MI segments per thread in flight.
-// Every region has one writer per step: wave 0 owns the bands and J(w),
-// J(w+1); the strips own the rest.
-template <int R, int NT, int MI>
+// ce(w+1) (chunks of 64 columns, loaded row-fastest and turned into one
+// column per lane through the wave's LDS tile) and to the rows above b0(w-1)
+// (chunks of 64 rows).  Each region has one writer per step.  Memory access
+// is through buffer loads/stores whose masked lanes use an offset past the
+// matrix (read as zero, stores dropped), so that no branch splits a batch of
+// loads; rows outside il .. iu are masked.
+template <int R, int NT, int NC>
 __global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm, int* info,
                                                         int maxit_per_row, QrLog log) {
-  constexpr int LDB = R + 3;
-  constexpr int PF = ((R + 1) * (R + 1) + 63) / 64;
-  constexpr int SW = NT / 64 - 1, RP = R + 1;
-  constexpr int JR = 2 * R + 1, JL = (R * JR + 63) / 64;  // J column rows; loads per lane
-  constexpr int JS = 2 * R - 1, JSL = (R * JS + 63) / 64;  // J rows stored per step
+  constexpr int GAP = 2, GP = GAP * (NC - 1);
+  constexpr int RS = R + GP + 1;   // rows spanned by a window's rotations
+  constexpr int NR = NC * R;       // rotation entries per window
+  constexpr int LDB = R + GP + 3;  // band side
+  constexpr int FS = R + 1;        // side of a band's fresh block, at most
+  constexpr int PF = (FS * FS + 63) / 64;
+  constexpr int SW = NT / 64 - 1;  // strip waves
+  constexpr int JR = 2 * R + GP + 1, JL = (R * JR + 63) / 64;  // J column rows; loads per lane
+  constexpr int JS = 2 * R - 1, JSL = (R * JS + 63) / 64;      // J rows stored per step
   constexpr int BL = (LDB * LDB + 63) / 64;
+  constexpr int HALF = 64 / NC;  // lanes per chain in the chase
   __shared__ zc band[2][LDB * LDB];
-  __shared__ zc stile[SW * MI * 64 * RP];  // each strip wave's transpose tiles
-  __shared__ zc jcol[R * JR];              // wave 0: column l of J(w) at l * JR, 2R+1 rows
-  __shared__ double rc[2][R];  // rotations of the current and the previous window
-  __shared__ zc rs[2][R];
+  __shared__ zc stile[SW * 64 * RS];  // each strip wave's transpose tile
+  __shared__ zc jcol[R * JR];         // wave 0: column l of J(w) at l * JR
+  __shared__ double rc[2][NR];        // rotations of the current and the previous window
+  __shared__ zc rs[2][NR];
   __shared__ int s_il;
-  __shared__ zc s_mu;
+  __shared__ zc s_mu[2];
   const int b = blockIdx.x;
   zc* T = Tm.p + (long long)b * Tm.stride;
   const QrLogM L = qr_log_of(log, b);
   const long long ld = Tm.ld;
   const unsigned tbytes = (unsigned)(ld * n * 16);  // below 2^31 (checked by the host)
   const __amdgpu_buffer_rsrc_t trs = matrix_rsrc(T, tbytes);
+  auto off_of = [&](int i, int j) { return (unsigned)(i + (long long)j * ld) * 16u; };
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double eps = DBL_EPSILON;
   const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
@@ -1095,12 +1120,21 @@ __global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm,
       __syncthreads();
       continue;
     }
-    if (nsw == log.scap || (long long)used + (iu - il) > log.limit) {
+    if (nsw + 1 > log.scap || (long long)used + (iu - il) > log.limit) {
+      // log full: stop before this pass; the resumed kernel repeats the
+      // deflation test and carries on (a fresh launch always has room for
+      // one sweep, so every round makes progress)
       stopped = true;
       break;
     }
-    ++iter;
-    ++total;
+    // two chained sweeps where the block is large, neither sweep takes an
+    // exceptional shift and the log has room for both
+    const bool dbl = NC == 2 && iu - il >= 2 * R && iter + 1 != 10 && iter + 2 != 10 &&
+                     iter + 1 != 20 && iter + 2 != 20 && nsw + 2 <= log.scap &&
+                     (long long)used + 2LL * (iu - il) <= log.limit;
+    const int nch = dbl ? 2 : 1;
+    iter += nch;
+    total += nch;
     if (total > maxit) {
       if (tid == 0 && info) info[b] = 1;
       failed = true;
@@ -1116,50 +1150,58 @@ __global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm,
       else
         mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
                        T[iu + (iu - 1) * ld], T[iu + iu * ld]);
-      s_mu = mu;
-      L.il[nsw] = il;
-      L.iu[nsw] = iu;
-      L.off[nsw] = used;
+      // both chains take the Wilkinson shift: two sweeps with it need about
+      // 13% more sweeps than sweeps with the shift updated in between, and
+      // far fewer than with the trailing 2x2 block's other eigenvalue for
+      // the second sweep (36% more)
+      s_mu[0] = mu;
+      s_mu[1] = mu;
+      for (int c = 0; c < nch; ++c) {
+        L.il[nsw + c] = il;
+        L.iu[nsw + c] = iu;
+        L.off[nsw + c] = used + c * (iu - il);
+      }
     }
     __syncthreads();
-    const int nw = (iu - il + R - 1) / R;
+    const int nw = (iu - il + (dbl ? GAP : 0) + R - 1) / R;
     auto wk0 = [&](int x) { return il + x * R; };
-    auto wk1 = [&](int x) { return min(il + x * R + R, iu); };
     auto wce = [&](int x) { return x < nw ? min(il + x * R + R + 1, iu) : iu; };
-    auto wb0 = [&](int x) { return x > 0 ? il + x * R - 1 : il; };
+    auto wb0 = [&](int x) { return x > 0 ? il + x * R - GP - 1 : il; };
+    // chain c's rotation at step t of the window starting at k0 is active
+    auto active = [&](int k0, int t, int c) {
+      const int kk = k0 + t - c * GAP;
+      return (c == 0 || dbl) && kk >= il && kk <= iu - 1;
+    };
     for (int w = 0; w <= nw; ++w) {
       const bool cur = w < nw, prev = w > 0;
-      const int k0 = wk0(w), k1 = cur ? wk1(w) : k0, nr = k1 - k0;
-      const int b0 = wb0(w), ce = wce(w), nb = ce - b0 + 1;
-      const int pk0 = prev ? wk0(w - 1) : 0, pnr = prev ? wk1(w - 1) - pk0 : 0;
-      const int pb0 = prev ? wb0(w - 1) : 0, pce = prev ? wce(w - 1) : 0;
+      const int k0 = wk0(w), b0 = wb0(w), ce = wce(w), nb = ce - b0 + 1;
+      const int pk0 = prev ? wk0(w - 1) : 0, pb0 = prev ? wb0(w - 1) : 0;
+      const int pce = prev ? wce(w - 1) : 0;
       const double* prc = rc[(w + 1) & 1];  // window w-1's rotations
       const zc* prs = rs[(w + 1) & 1];
       zc* cb = band[w & 1];
       const zc* pbnd = band[(w + 1) & 1];
       if (wave == 0) {
-        // loads for window w+1: rows pk0 .. pk0+2R of J(w+1) (in step 0 rows
-        // k0 .. k0+R, at R .. 2R), element e = column e / JR, row e % JR
+        // loads for window w+1: J(w+1), rows k0 - R - GP + q
         const int ce1 = wce(w + 1);
         const int nj1 = cur ? ce1 - ce : 0;
-        const int jr0 = prev ? pk0 : k0 - R;  // row of element q = 0
+        const int jr0 = k0 - R - GP;
         zc yv[JL];
 #pragma unroll
         for (int i = 0; i < JL; ++i) {
-          const int e = lane + 64 * i, col = e / JR, q = e % JR;
-          const bool ok = col < nj1 && (prev || q >= R);
-          yv[i] = buf_load(
-              trs, ok ? (unsigned)((jr0 + q) + (long long)(ce + 1 + col) * ld) * 16u : tbytes);
+          const int e = lane + 64 * i, col = e / JR, row = jr0 + e % JR;
+          const bool ok = col < nj1 && row >= il && row <= iu;
+          yv[i] = buf_load(trs, ok ? off_of(row, ce + 1 + col) : tbytes);
         }
+        // band w+1's fresh block: rows/columns ce .. ce1, less the corner
         zc pf[PF];
-        const int m1 = ce1 - k1;  // band w+1's fresh block: rows/columns k1+1 .. ce1
+        const int m1 = ce1 - ce + 1;
         if (w + 1 < nw) {
 #pragma unroll
           for (int p = 0; p < PF; ++p) {
-            const int idx = lane + 64 * p;
-            const int i = idx % m1, j = idx / m1;
+            const int idx = lane + 64 * p, i = idx % m1, j = idx / m1;
             pf[p] = buf_load(trs, idx < m1 * m1 && idx != 0 && i <= j + 1
-                                      ? (unsigned)((k1 + 1 + i) + (long long)(k1 + 1 + j) * ld) * 16u
+                                      ? off_of(ce + i, ce + j)
                                       : tbytes);
           }
         }
@@ -1170,46 +1212,49 @@ __global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm,
           for (int p = 0; p < BL; ++p) {
             const int idx = lane + 64 * p, i = idx % pnb, j = idx / pnb;
             const bool ok = idx < pnb * pnb && i <= j + 1;
-            buf_store(trs, ok ? (unsigned)((pb0 + i) + (long long)(pb0 + j) * ld) * 16u : tbytes,
-                      pbnd[ok ? i + j * LDB : 0]);
+            buf_store(trs, ok ? off_of(pb0 + i, pb0 + j) : tbytes, pbnd[ok ? i + j * LDB : 0]);
           }
-          if (lane < pnr) {
-            L.c[used + (pk0 - il) + lane] = prc[lane];
-            L.s[used + (pk0 - il) + lane] = prs[lane];
+          if (lane < NR) {
+            const int t = lane / NC, c = lane % NC;
+            if (active(pk0, t, c)) {
+              const int kk = pk0 + t - c * GAP;
+              L.c[used + c * (iu - il) + (kk - il)] = prc[lane];
+              L.s[used + c * (iu - il) + (kk - il)] = prs[lane];
+            }
           }
         }
         if (cur && prev) {
-          // J(w): column l in jcol holds rows pk0-R .. pk0+R; window w-1's
-          // rotations on rows pk0 .. pk0+R, rows above band w to memory, the
-          // two rows inside it (k0-1, k0) into LDS
+          // J(w): jcol row q is row k0 - 2R - GP + q; window w-1's rotations
+          // on q = R .. 2R + GP, rows q < 2R - 1 to memory, the rest (rows
+          // b0 .. k0) into band w
           const int nj = ce - pce;
           if (lane < nj) {
-            zc xr[R + 1];
+            zc xr[RS];
 #pragma unroll
-            for (int q = 0; q <= R; ++q) xr[q] = jcol[lane * JR + R + q];
-            rotate_reg_rows<R, 0>(xr, pnr, prc, prs);
+            for (int i = 0; i < RS; ++i) xr[i] = jcol[lane * JR + R + i];
+            rotate_window<R, NC, GAP>(xr, false, prc, prs);
 #pragma unroll
-            for (int q = 0; q < R - 1; ++q) jcol[lane * JR + R + q] = xr[q];
-            cb[0 + (pce + 1 + lane - b0) * LDB] = xr[R - 1];
-            cb[1 + (pce + 1 + lane - b0) * LDB] = xr[R];
+            for (int i = 0; i < R - 1; ++i) jcol[lane * JR + R + i] = xr[i];
+#pragma unroll
+            for (int i = R - 1; i < RS; ++i)
+              cb[(i - (R - 1)) + (pce + 1 + lane - b0) * LDB] = xr[i];
           }
 #pragma unroll
           for (int i = 0; i < JSL; ++i) {
-            const int e = lane + 64 * i, col = e / JS, q = e % JS;
-            const bool ok = col < nj && (w >= 2 || q >= R);
-            buf_store(trs, ok ? (unsigned)((pk0 - R + q) + (long long)(pce + 1 + col) * ld) * 16u
-                              : tbytes,
-                      jcol[ok ? col * JR + q : 0]);
+            const int e = lane + 64 * i, col = e / JS, q = e % JS, row = k0 - 2 * R - GP + q;
+            const bool ok = col < nj && row >= il;
+            buf_store(trs, ok ? off_of(row, pce + 1 + col) : tbytes, jcol[ok ? col * JR + q : 0]);
           }
-          // overlap with band w-1: rows/columns k0-1 .. k0+1
-          if (lane < 9) {
-            const int i = lane % 3, j = lane / 3, o = b0 - pb0;
+          // overlap with band w-1: rows/columns b0 .. pce
+          const int no = pce - b0 + 1, o = b0 - pb0;
+          if (lane < no * no) {
+            const int i = lane % no, j = lane / no;
             cb[i + j * LDB] = pbnd[(o + i) + (o + j) * LDB];
           }
-          // rows k0+2 .. ce of columns k0-1, k0
-          const int mz = ce - k0 - 1;
-          for (int idx = lane; idx < 2 * mz; idx += 64)
-            cb[(3 + idx % mz) + (idx / mz) * LDB] = zmake(0, 0);
+          // rows pce+1 .. ce of columns b0 .. pce-1 (below the subdiagonal)
+          const int mzr = ce - pce, mzc = pce - b0;
+          for (int idx = lane; idx < mzr * mzc; idx += 64)
+            cb[(pce + 1 - b0 + idx % mzr) + (idx / mzr) * LDB] = zmake(0, 0);
         } else if (cur) {
           for (int idx = lane; idx < nb * nb; idx += 64) {
             const int i = idx % nb, j = idx / nb;
@@ -1222,149 +1267,144 @@ __global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm,
         if (cur) {
           double* crc = rc[w & 1];
           zc* crs = rs[w & 1];
-          for (int k = k0; k < k1; ++k) {
-            // every lane computes the same rotation (no broadcast)
-            double c;
-            zc sr, r, f, g;
-            if (k == il) {
-              f = zsub(cb[(k - b0) + (k - b0) * LDB], s_mu);
-              g = cb[(k + 1 - b0) + (k - b0) * LDB];
-            } else {
-              f = cb[(k - b0) + (k - 1 - b0) * LDB];
-              g = cb[(k + 1 - b0) + (k - 1 - b0) * LDB];
-            }
-            givens_rsq(f, g, c, sr, r);
-            if (lane == 0) {
-              if (k > il) {
-                cb[(k - b0) + (k - 1 - b0) * LDB] = r;
-                cb[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
+          const int ch = lane / HALF, hl = lane % HALF;
+          for (int t = 0; t < R; ++t) {
+            // every lane of a chain computes that chain's rotation
+            const int kk = k0 + t - ch * GAP;
+            const bool act = active(k0, t, ch);
+            double c = 1.0;
+            zc sr = zmake(0, 0), r = zmake(0, 0);
+            if (act) {
+              zc f, g;
+              if (kk == il) {
+                f = zsub(cb[(kk - b0) + (kk - b0) * LDB], s_mu[ch]);
+                g = cb[(kk + 1 - b0) + (kk - b0) * LDB];
+              } else {
+                f = cb[(kk - b0) + (kk - 1 - b0) * LDB];
+                g = cb[(kk + 1 - b0) + (kk - 1 - b0) * LDB];
               }
-              crc[k - k0] = c;
-              crs[k - k0] = sr;
+              givens_rsq(f, g, c, sr, r);
             }
-            const zc sc = zconj(sr);
-            for (int j = k + lane; j <= ce; j += 64) {
-              zc* p0 = &cb[(k - b0) + (j - b0) * LDB];
-              const zc x = p0[0], y = p0[1];
-              p0[0] = zadd(zscale(x, c), zmul(sr, y));
-              p0[1] = zsub(zscale(y, c), zmul(sc, x));
+            if (hl == 0) {
+              if (act && kk > il) {
+                cb[(kk - b0) + (kk - 1 - b0) * LDB] = r;
+                cb[(kk + 1 - b0) + (kk - 1 - b0) * LDB] = zmake(0, 0);
+              }
+              crc[NC * t + ch] = c;
+              crs[NC * t + ch] = sr;
+            }
+            if (act) {
+              const zc sc = zconj(sr);
+              for (int j = kk + hl; j <= ce; j += HALF) {
+                zc* p0 = &cb[(kk - b0) + (j - b0) * LDB];
+                const zc x = p0[0], y = p0[1];
+                p0[0] = zadd(zscale(x, c), zmul(sr, y));
+                p0[1] = zsub(zscale(y, c), zmul(sc, x));
+              }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            const int ie = min(k + 2, iu);
-            for (int i = b0 + lane; i <= ie; i += 64) {
-              zc* q0 = &cb[(i - b0) + (k - b0) * LDB];
-              const zc x = q0[0], y = q0[LDB];
-              q0[0] = zadd(zscale(x, c), zmul(sc, y));
-              q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+            if (act) {
+              const zc sc = zconj(sr);
+              const int ie = min(kk + 2, iu);
+              for (int i = b0 + hl; i <= ie; i += HALF) {
+                zc* q0 = &cb[(i - b0) + (kk - b0) * LDB];
+                const zc x = q0[0], y = q0[LDB];
+                q0[0] = zadd(zscale(x, c), zmul(sc, y));
+                q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+              }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
           }
         }
         if (nj1 > 0) {
-          // J(w+1) into jcol; window w-1's rotations on its rows pk0 .. pk0+R
+          // J(w+1) into jcol; window w-1's rotations on its rows q = 0 .. RS-1
 #pragma unroll
           for (int i = 0; i < JL; ++i)
             if (lane + 64 * i < R * JR) jcol[lane + 64 * i] = yv[i];
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
           __builtin_amdgcn_wave_barrier();
           if (prev && lane < nj1) {
-            zc xr[R + 1];
+            zc xr[RS];
 #pragma unroll
-            for (int q = 0; q <= R; ++q) xr[q] = jcol[lane * JR + q];
-            rotate_reg_rows<R, 0>(xr, pnr, prc, prs);
+            for (int i = 0; i < RS; ++i) xr[i] = jcol[lane * JR + i];
+            rotate_window<R, NC, GAP>(xr, false, prc, prs);
 #pragma unroll
-            for (int q = 0; q <= R; ++q) jcol[lane * JR + q] = xr[q];
+            for (int i = 0; i < RS; ++i) jcol[lane * JR + i] = xr[i];
           }
         }
         if (w + 1 < nw) {
-          // band w+1's rows/columns k1+1 .. ce1, less the corner (band w-1's
-          // buffer, read for the last time above)
+          // band w+1's fresh block (into band w-1's buffer, read for the last
+          // time above); its corner row/column ce sits at ce - b0(w+1)
           zc* nbnd = band[(w + 1) & 1];
+          const int f0 = ce - wb0(w + 1);
 #pragma unroll
           for (int p = 0; p < PF; ++p) {
             const int idx = lane + 64 * p;
-            if (idx < m1 * m1 && idx != 0) nbnd[(2 + idx % m1) + (2 + idx / m1) * LDB] = pf[p];
+            if (idx < m1 * m1 && idx != 0)
+              nbnd[(f0 + idx % m1) + (f0 + idx / m1) * LDB] = pf[p];
           }
         }
       } else if (prev) {
-        // chunks of 64 columns of the right strip (loaded row-fastest, turned
-        // into one column per lane through this wave's LDS tile) and of 64
-        // rows of the top strip (one row per lane), MI chunks in flight
+        // window w-1's rotations: chunks of 64 columns of the right strip
+        // (columns r0 .. iu) and of 64 rows of the top strip (rows il ..
+        // pb0-1); a segment covers rows/columns pk0 - GP .. pk0 + R
         const int sw = wave - 1;
-        const int r0 = wce(w + 1) + 1;  // right strip: columns r0 .. iu
+        const int r0 = wce(w + 1) + 1;
         const int nright = max(iu - r0 + 1, 0);
-        const int ntop = max(pb0 - il, 0);  // top strip: rows il .. pb0-1
-        const int nrc = (nright + 63) / 64, nch = nrc + (ntop + 63) / 64;
-        zc* tile = stile + sw * (MI * 64 * RP);
-        for (int c0 = sw; c0 < nch; c0 += SW * MI) {
-          zc v[MI][RP];
-          unsigned off[MI][RP];
+        const int ntop = max(pb0 - il, 0);
+        const int nrc = (nright + 63) / 64, nch2 = nrc + (ntop + 63) / 64;
+        const int s0 = pk0 - GP;
+        zc* tile = stile + sw * (64 * RS);
+        for (int ch = sw; ch < nch2; ch += SW) {
+          zc v[RS];
+          unsigned off[RS];
+          const bool right = ch < nrc;
+          if (right) {
+            const int cb0 = r0 + ch * 64, ncol = min(64, iu - cb0 + 1);
 #pragma unroll
-          for (int mi = 0; mi < MI; ++mi) {
-            const int ch = c0 + mi * SW;
-            if (ch < nrc) {
-              const int cb = r0 + ch * 64, ncol = min(64, iu - cb + 1);
-#pragma unroll
-              for (int i = 0; i < RP; ++i) {
-                const int e = lane + 64 * i, col = e / RP, row = e % RP;
-                off[mi][i] = col < ncol && row <= pnr
-                                 ? (unsigned)((pk0 + row) + (long long)(cb + col) * ld) * 16u
-                                 : tbytes;
-              }
-            } else {
-              const int row = il + (ch - nrc) * 64 + lane;
-              const bool okr = ch < nch && row < pb0;
-#pragma unroll
-              for (int q = 0; q < RP; ++q)
-                off[mi][q] = okr && q <= pnr
-                                 ? (unsigned)(row + (long long)(pk0 + q) * ld) * 16u
-                                 : tbytes;
+            for (int i = 0; i < RS; ++i) {
+              const int e = lane + 64 * i, col = e / RS, row = s0 + e % RS;
+              off[i] = col < ncol && row >= il && row <= iu ? off_of(row, cb0 + col) : tbytes;
             }
+          } else {
+            const int row = il + (ch - nrc) * 64 + lane;
 #pragma unroll
-            for (int i = 0; i < RP; ++i) v[mi][i] = buf_load(trs, off[mi][i]);
-          }
-#pragma unroll
-          for (int mi = 0; mi < MI; ++mi)
-            if (c0 + mi * SW < nrc) {
-#pragma unroll
-              for (int i = 0; i < RP; ++i) tile[mi * 64 * RP + lane + 64 * i] = v[mi][i];
-            }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-#pragma unroll
-          for (int mi = 0; mi < MI; ++mi) {
-            const bool right = c0 + mi * SW < nrc;
-            if (right) {
-#pragma unroll
-              for (int q = 0; q < RP; ++q) v[mi][q] = tile[mi * 64 * RP + lane * RP + q];
-            }
-            rotate_segment<R>(v[mi], pnr, !right, prc, prs);
-            if (right) {
-#pragma unroll
-              for (int q = 0; q < RP; ++q) tile[mi * 64 * RP + lane * RP + q] = v[mi][q];
+            for (int q = 0; q < RS; ++q) {
+              const int col = s0 + q;
+              off[q] = row < pb0 && col >= il && col <= iu ? off_of(row, col) : tbytes;
             }
           }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
 #pragma unroll
-          for (int mi = 0; mi < MI; ++mi) {
-            if (c0 + mi * SW < nrc) {
+          for (int i = 0; i < RS; ++i) v[i] = buf_load(trs, off[i]);
+          if (right) {
 #pragma unroll
-              for (int i = 0; i < RP; ++i) v[mi][i] = tile[mi * 64 * RP + lane + 64 * i];
-            }
+            for (int i = 0; i < RS; ++i) tile[lane + 64 * i] = v[i];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
-            for (int i = 0; i < RP; ++i) buf_store(trs, off[mi][i], v[mi][i]);
+            for (int q = 0; q < RS; ++q) v[q] = tile[lane * RS + q];
           }
+          rotate_window<R, NC, GAP>(v, !right, prc, prs);
+          if (right) {
+#pragma unroll
+            for (int q = 0; q < RS; ++q) tile[lane * RS + q] = v[q];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int i = 0; i < RS; ++i) v[i] = tile[lane + 64 * i];
+          }
+#pragma unroll
+          for (int i = 0; i < RS; ++i) buf_store(trs, off[i], v[i]);
           __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
           __builtin_amdgcn_wave_barrier();
         }
       }
       __syncthreads();
     }
-    used += iu - il;
-    ++nsw;
+    used += nch * (iu - il);
+    nsw += nch;
   }
   if (tid == 0) {
     log.nsw[b] = nsw;
@@ -1491,8 +1531,11 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
       if (mode == 'o' || (long long)T.ld * n * 16 >= (1LL << 31))
         hipLaunchKernelGGL((schur_defer_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z,
                            info, maxit_per_row, log);
-      else
+      else if (mode == 's')
         hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 1>), dim3(batch), dim3(256), 0, s, n, T,
+                           Z, info, maxit_per_row, log);
+      else
+        hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 2>), dim3(batch), dim3(256), 0, s, n, T,
                            Z, info, maxit_per_row, log);
       if ((e = (int)hipGetLastError())) return e;
       hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
