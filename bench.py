@@ -37,7 +37,7 @@ METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| 
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR kernels at 240
 # matrices per launch, scaled by the counters' reading on a copy of known size
 # (profiles/qr_phase_21x21_pmc_traffic.md).
-QR_PHASE_HBM_BYTES_PER_MATRIX = 4567.2e9 / 240
+QR_PHASE_HBM_BYTES_PER_MATRIX = 3492.8e9 / 240
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -261,8 +261,9 @@ def main():
     per_launch_mats = st["n_eig_matrices"] / launches
     avg_ms = st["ms_eig_schur"] / launches
     achieved = f_qr * per_launch_mats / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    roofline = {"kernel": "QR phase: schur_defer_kernel + 2 x qr_apply_kernel (complex "
-                          "single-shift QR sweeps, Schur form and Schur vectors)",
+    roofline = {"kernel": "QR phase: schur_pipe_kernel + 2 x qr_apply_kernel (complex "
+                          "single-shift QR sweeps, two chained per pass; Schur form and "
+                          "Schur vectors)",
                 "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats,
