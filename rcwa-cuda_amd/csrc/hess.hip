@@ -24,7 +24,7 @@
 
 namespace {
 
-constexpr int NB = 32;
+constexpr int NB = 64;
 constexpr int PT = 1024;  // panel threads
 
 __device__ __forceinline__ zc shfl_down_z(zc v, int off) {
@@ -129,7 +129,23 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
     zc yacc[(4096 + PT - 1) / PT];
 #pragma unroll
     for (int r = 0; r < (4096 + PT - 1) / PT; ++r) yacc[r] = zmake(0, 0);
-    for (int c = j + 1; c < n; ++c) {
+    // eight columns per iteration, their loads issued together
+    int c = j + 1;
+    for (; c + 7 < n; c += 8) {
+      const zc* col = A + (long long)c * lda;
+#pragma unroll
+      for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
+        const int i = tid + r * PT;
+        if (i < n) {
+          zc a[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) a[q] = col[i + q * lda];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) yacc[r] = zfma(a[q], bcol[c + q], yacc[r]);
+        }
+      }
+    }
+    for (; c < n; ++c) {
       const zc vc = bcol[c];
       const zc* col = A + (long long)c * lda;
 #pragma unroll
