@@ -150,11 +150,13 @@ def c5_bench():
     import workloads as W
     p = W.c5_large()
     N = 41 * 41
+    os.environ["RCWA_PROGRESS"] = "1"  # one line per layer batch: the solve takes minutes
     ctx = G.Context(0)
     ctx.set_timing(True)
     t0 = time.perf_counter()
     r = ctx.solve([p])[0]
     wall = time.perf_counter() - t0
+    del os.environ["RCWA_PROGRESS"]
     st = ctx.stats()
     ctx.close()
     flops = (2384.0 * 50 + 1621.0) * N ** 3
@@ -303,6 +305,14 @@ def main():
         dT = max(np.abs(a[1] - b[1]).max() for ra, rb in zip(results, sres) for a, b in zip(ra, rb))
         out["split"] = {"solves_per_s": solves / selapsed, "speedup_vs_fp64": elapsed / selapsed,
                         "max_dR_vs_fp64": float(dR), "max_dT_vs_fp64": float(dT)}
+
+    # the stream and the large structure below use contexts of their own:
+    # release this one's HBM arena and the sweep's grids first
+    ctx.close()
+    for d in er_devs:
+        hip.free(d)
+    hip.free(ur_dev)
+    er_devs = []
 
     if args.serve:
         progress("serving stream of %d requests" % args.serve)

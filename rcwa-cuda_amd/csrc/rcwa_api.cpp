@@ -16,6 +16,8 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -454,6 +456,10 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   }
   CKH(hipEventRecord(c.ev[1], s));
   // ---- SMatrixLayer for every layer (main.cpp:151)
+  // RCWA_PROGRESS set: a line on stderr as each layer batch completes (for
+  // long single solves; the wait it adds is one stream synchronisation per
+  // batch)
+  const bool progress = getenv("RCWA_PROGRESS") != nullptr;
   for (int l0 = 0; l0 < TL; l0 += Bl) {
     const int b = std::min(Bl, TL - l0);
     st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
@@ -463,6 +469,11 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
                      ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
                      info_eig + l0, info_sing + l0);
     if (st) return st;
+    if (progress) {
+      CKH(hipStreamSynchronize(s));
+      fprintf(stderr, "[rcwa] layer matrices %d/%d done\n", l0 + b, TL);
+      fflush(stderr);
+    }
   }
   CKH(hipEventRecord(c.ev[4], s));
   // ---- fold (main.cpp:143-170) for the two source columns
