@@ -71,11 +71,19 @@ typedef struct {
 } rcwa_params;
 
 /* rcwa.hpp:116-128 (struct Results).  R and T are caller-allocated arrays of
- * (2*Ny_harmonics+1)*(2*Nx_harmonics+1) doubles (Q x P, column-major). */
+ * (2*Ny_harmonics+1)*(2*Nx_harmonics+1) = PQ doubles (Q x P, column-major).
+ * r and t (optional, NULL to skip) receive the per-order complex amplitudes
+ * of the reflected and transmitted fields, the 3PQ-vectors of
+ * ComputeReflectedField / ComputeTransmittedField (rcwa.cpp:526-561): the x
+ * components of every order (harmonic index m = ix*Q + iy), then y, then z.
+ * The reference keeps them inside main.cpp:184-190; exposing them is a
+ * widening of the interface for callers that need phases (Jones matrices). */
 typedef struct {
   double* R;
   double* T;
   double R_tot, T_tot;
+  rcwa_c128* r;
+  rcwa_c128* t;
 } rcwa_results;
 
 /* One independent solve: a device, its media and nsrc sources.  Sources

@@ -146,6 +146,8 @@ Efficiencies efficiencies(double theta, double pte, double ptm, const WaveVector
     e.R_tot += e.R[i];
     e.T_tot += e.T[i];
   }
+  e.r = std::move(r);
+  e.t = std::move(t);
   return e;
 }
 

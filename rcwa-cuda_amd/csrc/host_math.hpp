@@ -47,6 +47,7 @@ std::vector<cplx> end_smatrix_diagonals(int which, const std::vector<cplx>& Kx,
 struct Efficiencies {
   std::vector<double> R, T;  // PQ each, harmonic index order (= Q x P column-major)
   double R_tot = 0, T_tot = 0;
+  std::vector<cplx> r, t;    // 3PQ each: the reflected / transmitted fields (x, y, z)
 };
 
 // ComputeSourceModeCoeff + Compute{Reflected,Transmitted}Field +

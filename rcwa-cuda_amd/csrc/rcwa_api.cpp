@@ -596,6 +596,8 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
       rcwa_results& o = j.out[si];
       if (o.R) std::memcpy(o.R, e.R.data(), N * sizeof(double));
       if (o.T) std::memcpy(o.T, e.T.data(), N * sizeof(double));
+      if (o.r) std::memcpy(o.r, e.r.data(), 3 * (size_t)N * sizeof(rcwa_c128));
+      if (o.t) std::memcpy(o.t, e.t.data(), 3 * (size_t)N * sizeof(rcwa_c128));
       o.R_tot = e.R_tot;
       o.T_tot = e.T_tot;
       if (!std::isfinite(e.R_tot) || !std::isfinite(e.T_tot))

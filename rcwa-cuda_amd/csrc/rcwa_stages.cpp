@@ -356,6 +356,8 @@ rcwa_status rcwa_efficiencies(const rcwa_source* src, const rcwa_params* p, cons
       rcwa_host::efficiencies(src->theta, src->pte, src->ptm, w, m, a.data(), b.data());
   if (out->R) std::memcpy(out->R, e.R.data(), N * sizeof(double));
   if (out->T) std::memcpy(out->T, e.T.data(), N * sizeof(double));
+  if (out->r) std::memcpy(out->r, e.r.data(), 3 * (size_t)N * sizeof(rcwa_c128));
+  if (out->t) std::memcpy(out->t, e.t.data(), 3 * (size_t)N * sizeof(rcwa_c128));
   out->R_tot = e.R_tot;
   out->T_tot = e.T_tot;
   return RCWA_OK;

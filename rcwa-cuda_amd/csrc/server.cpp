@@ -45,6 +45,7 @@ struct Request {
   rcwa_params par{};
   std::vector<rcwa_source> src;
   std::vector<double> R, T;
+  std::vector<rcwa_c128> ra, ta;  // per-order amplitudes, 3N per source
   std::vector<rcwa_results> res;
   int N = 0;
   std::tuple<int, int, int, int> key;
@@ -149,6 +150,10 @@ rcwa_status fill(rcwa_server* s, Request* r, rcwa_results* out, rcwa_ticket_info
     for (size_t i = 0; i < r->src.size(); ++i) {
       if (out[i].R) std::memcpy(out[i].R, r->R.data() + i * r->N, r->N * sizeof(double));
       if (out[i].T) std::memcpy(out[i].T, r->T.data() + i * r->N, r->N * sizeof(double));
+      if (out[i].r)
+        std::memcpy(out[i].r, r->ra.data() + 3 * i * r->N, 3 * r->N * sizeof(rcwa_c128));
+      if (out[i].t)
+        std::memcpy(out[i].t, r->ta.data() + 3 * i * r->N, 3 * r->N * sizeof(rcwa_c128));
       out[i].R_tot = r->res[i].R_tot;
       out[i].T_tot = r->res[i].T_tot;
     }
@@ -218,9 +223,13 @@ rcwa_status rcwa_server_submit(rcwa_server* s, const rcwa_problem* p, uint64_t* 
   r->N = (2 * r->par.Nx_harmonics + 1) * (2 * r->par.Ny_harmonics + 1);
   r->R.assign((size_t)p->nsrc * r->N, 0.0);
   r->T.assign((size_t)p->nsrc * r->N, 0.0);
+  r->ra.assign(3 * (size_t)p->nsrc * r->N, rcwa_c128{0, 0});
+  r->ta.assign(3 * (size_t)p->nsrc * r->N, rcwa_c128{0, 0});
   r->res.resize(p->nsrc);
   for (int i = 0; i < p->nsrc; ++i)
-    r->res[i] = rcwa_results{r->R.data() + (size_t)i * r->N, r->T.data() + (size_t)i * r->N, 0, 0};
+    r->res[i] = rcwa_results{r->R.data() + (size_t)i * r->N, r->T.data() + (size_t)i * r->N, 0, 0,
+                             r->ra.data() + 3 * (size_t)i * r->N,
+                             r->ta.data() + 3 * (size_t)i * r->N};
   r->key = {r->par.Nx_harmonics, r->par.Ny_harmonics, d.Nx, d.Ny};
   {
     std::lock_guard<std::mutex> lk(s->mu);

@@ -51,7 +51,8 @@ class Params(ctypes.Structure):
 
 class Results(ctypes.Structure):
     _fields_ = [("R", ctypes.c_void_p), ("T", ctypes.c_void_p),
-                ("R_tot", ctypes.c_double), ("T_tot", ctypes.c_double)]
+                ("R_tot", ctypes.c_double), ("T_tot", ctypes.c_double),
+                ("r", ctypes.c_void_p), ("t", ctypes.c_void_p)]
 
 
 class Problem(ctypes.Structure):
@@ -183,11 +184,14 @@ class Context:
         _check(lib().rcwa_ctx_reset_stats(self.h))
 
     # ---------------------------------------------------------------- solves
-    def solve(self, probs, precision=RCWA_FP64, device_grids=None, grid_shape=None):
+    def solve(self, probs, precision=RCWA_FP64, device_grids=None, grid_shape=None,
+              fields=False):
         """Solve a list of workloads.Problem.  Returns a list (per problem) of
-        (R[npol][Q][P], T[npol][Q][P], tot[npol][2]).  device_grids, if given,
-        is a list of (er_dev_ptr, ur_dev_ptr) for HBM-resident grids of shape
-        [len(p.thick)][grid_shape[0]][grid_shape[1]]."""
+        (R[npol][Q][P], T[npol][Q][P], tot[npol][2]), with fields=True
+        followed by the per-order complex amplitudes r, t [npol][3][PQ]
+        (x, y, z components; harmonic index m = ix*Q + iy).  device_grids, if
+        given, is a list of (er_dev_ptr, ur_dev_ptr) for HBM-resident grids of
+        shape [len(p.thick)][grid_shape[0]][grid_shape[1]]."""
         keep = []
         cprobs = (Problem * len(probs))()
         outs = []
@@ -199,7 +203,9 @@ class Context:
             npol = len(p.pols)
             R = np.zeros((npol, P * Q))
             T = np.zeros((npol, P * Q))
-            outs.append((R, T))
+            ra = np.zeros((npol, 3 * P * Q), np.complex128) if fields else None
+            ta = np.zeros((npol, 3 * P * Q), np.complex128) if fields else None
+            outs.append((R, T, ra, ta))
             if device_grids is not None:
                 er_p, ur_p = device_grids[i]
                 ondev = 1
@@ -213,7 +219,9 @@ class Context:
             keep += [d, par, srcs, p]
             cprobs[i] = Problem(ctypes.pointer(d), ctypes.pointer(par), srcs, npol)
             for j in range(npol):
-                res[k] = Results(R[j].ctypes.data, T[j].ctypes.data, 0.0, 0.0)
+                res[k] = Results(R[j].ctypes.data, T[j].ctypes.data, 0.0, 0.0,
+                                 ra[j].ctypes.data if fields else None,
+                                 ta[j].ctypes.data if fields else None)
                 k += 1
         _check(lib().rcwa_solve_batch(self.h, cprobs, len(probs), precision, res))
         out = []
@@ -221,11 +229,14 @@ class Context:
         for i, p in enumerate(probs):
             P, Q = 2 * p.hx + 1, 2 * p.hy + 1
             npol = len(p.pols)
-            R, T = outs[i]
+            R, T, ra, ta = outs[i]
             tot = np.array([[res[k + j].R_tot, res[k + j].T_tot] for j in range(npol)])
             k += npol
-            out.append((R.reshape(npol, P, Q).transpose(0, 2, 1),
-                        T.reshape(npol, P, Q).transpose(0, 2, 1), tot))
+            one = (R.reshape(npol, P, Q).transpose(0, 2, 1),
+                   T.reshape(npol, P, Q).transpose(0, 2, 1), tot)
+            if fields:
+                one += (ra.reshape(npol, 3, P * Q), ta.reshape(npol, 3, P * Q))
+            out.append(one)
         return out
 
     def last_source_columns(self, count):
@@ -351,12 +362,16 @@ class Server:
         self._meta[t.value] = (p.hx, p.hy, npol)
         return t.value
 
-    def _collect(self, fn, ticket):
+    def _collect(self, fn, ticket, fields=False):
         hx, hy, npol = self._meta[ticket]
         P, Q = 2 * hx + 1, 2 * hy + 1
         R = np.zeros((npol, P * Q))
         T = np.zeros((npol, P * Q))
-        res = (Results * npol)(*[Results(R[j].ctypes.data, T[j].ctypes.data, 0.0, 0.0)
+        ra = np.zeros((npol, 3 * P * Q), np.complex128)
+        ta = np.zeros((npol, 3 * P * Q), np.complex128)
+        res = (Results * npol)(*[Results(R[j].ctypes.data, T[j].ctypes.data, 0.0, 0.0,
+                                         ra[j].ctypes.data if fields else None,
+                                         ta[j].ctypes.data if fields else None)
                                  for j in range(npol)])
         info = TicketInfo()
         rc = fn(self.h, ctypes.c_uint64(ticket), res, ctypes.byref(info))
@@ -364,15 +379,20 @@ class Server:
             return None
         _check(rc)
         tot = np.array([[res[j].R_tot, res[j].T_tot] for j in range(npol)])
-        return (R.reshape(npol, P, Q).transpose(0, 2, 1), T.reshape(npol, P, Q).transpose(0, 2, 1),
-                tot, {"t_submit": info.t_submit, "t_start": info.t_start,
-                      "t_finish": info.t_finish, "batch_size": info.batch_size})
+        info_d = {"t_submit": info.t_submit, "t_start": info.t_start,
+                  "t_finish": info.t_finish, "batch_size": info.batch_size}
+        out = (R.reshape(npol, P, Q).transpose(0, 2, 1), T.reshape(npol, P, Q).transpose(0, 2, 1),
+               tot, info_d)
+        if fields:
+            out += (ra.reshape(npol, 3, P * Q), ta.reshape(npol, 3, P * Q))
+        return out
 
-    def wait(self, ticket):
-        return self._collect(lib().rcwa_server_wait, ticket)
+    def wait(self, ticket, fields=False):
+        """(R, T, tot, info), with fields=True followed by r, t [npol][3][PQ]."""
+        return self._collect(lib().rcwa_server_wait, ticket, fields)
 
-    def poll(self, ticket):
-        return self._collect(lib().rcwa_server_poll, ticket)
+    def poll(self, ticket, fields=False):
+        return self._collect(lib().rcwa_server_poll, ticket, fields)
 
     def cancel(self, ticket):
         _check(lib().rcwa_server_cancel(self.h, ctypes.c_uint64(ticket)))

@@ -77,7 +77,10 @@ def test_host_end_smatrix_matches_dense_restatement(which):
 
 def test_host_efficiencies_match_oracle_solve():
     """The tail (source, fields, efficiencies) applied to the oracle's own
-    global S-matrix reproduces the oracle's R and T."""
+    global S-matrix reproduces the oracle's R and T, and its per-order
+    amplitudes r, t (rcwa.cpp:526-561) those of the numpy restatement's
+    source and field steps on the same S-matrix; R and T follow from |r|^2
+    and |t|^2 by the efficiency formula (rcwa.cpp:574-583)."""
     import ctypes
     rng = np.random.default_rng(12)
     p = W.random_small(rng, hx=2, hy=1, L=2, grid=8)
@@ -93,10 +96,20 @@ def test_host_efficiencies_match_oracle_solve():
         src = G.Source(p.lam0, p.theta, p.phi, a, b)
         Rh = np.zeros(N)
         Th = np.zeros(N)
-        res = G.Results(Rh.ctypes.data, Th.ctypes.data, 0.0, 0.0)
+        rh = np.zeros(3 * N, np.complex128)
+        th = np.zeros(3 * N, np.complex128)
+        res = G.Results(Rh.ctypes.data, Th.ctypes.data, 0.0, 0.0, rh.ctypes.data, th.ctypes.data)
         G._check(G.lib().rcwa_efficiencies(ctypes.byref(src), ctypes.byref(par), G._ptr(k),
                                            G._ptr(Kx), G._ptr(Ky), G._ptr(Kzr), G._ptr(Kzt),
                                            G._ptr(s11c), G._ptr(s21c), ctypes.byref(res)))
         assert np.abs(Rh.reshape(2 * p.hx + 1, -1).T - R[i]).max() < 1e-13
         assert np.abs(Th.reshape(2 * p.hx + 1, -1).T - T[i]).max() < 1e-13
         assert abs(res.R_tot - tot[i][0]) < 1e-13 and abs(res.T_tot - tot[i][1]) < 1e-13
+        I = np.eye(2 * N, dtype=np.complex128)
+        csrc = NR.source_coeff(p.theta, a, b, k, N, I)
+        r_ref = NR.fields(Sg, csrc, Kx, Ky, Kzr, I, 0)
+        t_ref = NR.fields(Sg, csrc, Kx, Ky, Kzt, I, 1)
+        assert np.abs(rh - r_ref).max() < 1e-13 and np.abs(th - t_ref).max() < 1e-13
+        den = (k[2] / p.ur_ref).real
+        rr = (np.abs(rh.reshape(3, N)) ** 2).sum(axis=0)
+        assert np.abs((-Kzr / p.ur_ref).real / den * rr - Rh).max() < 1e-13

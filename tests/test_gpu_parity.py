@@ -5,6 +5,7 @@ absolute of the oracle; R + T = 1 within 1e-10 for lossless stacks."""
 import numpy as np
 import pytest
 
+import np_rcwa as NR
 import oracle_py as O
 import rcwa_amd as G
 import workloads as W
@@ -139,3 +140,39 @@ def test_line_grating_sweep_vs_oracle(ctx):
         ref = O.solve(p)
         dR, dT, dtot = _cmp(gpu, ref)
         assert max(dR, dT, dtot) < TOL, (p.lam0, dR, dT, dtot)
+
+
+def _oracle_amplitudes(p):
+    """Per-order r, t of the oracle's global S-matrix (the numpy
+    restatement's source and field steps, rcwa.cpp:469-561, W_ref = I)."""
+    _, _, _, Sg = O.solve(p, return_smatrix=True)
+    k, Kx, Ky, Kzr, Kzt = O.wave_vectors(p.Lx, p.Ly, p.lam0, p.theta, p.phi, p.hx, p.hy,
+                                         p.er_ref, p.ur_ref, p.er_trn, p.ur_trn)
+    N = Kx.size
+    I = np.eye(2 * N, dtype=np.complex128)
+    out = []
+    for a, b in p.pols:
+        csrc = NR.source_coeff(p.theta, a, b, k, N, I)
+        out.append((NR.fields(Sg, csrc, Kx, Ky, Kzr, I, 0).reshape(3, N),
+                    NR.fields(Sg, csrc, Kx, Ky, Kzt, I, 1).reshape(3, N)))
+    return out
+
+
+@pytest.mark.parametrize("seed,lossless", [(41, True), (42, False)])
+def test_amplitudes_vs_oracle(ctx, seed, lossless):
+    """Per-order complex amplitudes r, t (the fields of rcwa.cpp:526-561)
+    within 1e-9 of the oracle's, phases included, through the batched solve
+    and through the serving form."""
+    rng = np.random.default_rng(seed)
+    probs = [W.random_small(rng, hx=2, hy=2, L=3, grid=16, lossless=lossless) for _ in range(2)]
+    out = ctx.solve(probs, fields=True)
+    with G.Server(0, n_workers=1) as srv:
+        tickets = [srv.submit(p) for p in probs]
+        served = [srv.wait(t, fields=True) for t in tickets]
+    for p, g, s in zip(probs, out, served):
+        ref = _oracle_amplitudes(p)
+        for j, (r_ref, t_ref) in enumerate(ref):
+            assert np.abs(g[3][j] - r_ref).max() < TOL
+            assert np.abs(g[4][j] - t_ref).max() < TOL
+            assert np.abs(s[4][j] - r_ref).max() < TOL
+            assert np.abs(s[5][j] - t_ref).max() < TOL
