@@ -219,6 +219,36 @@ rcwa_status rcwa_convmat_batched(rcwa_ctx* ctx, const rcwa_c128* grids_dev, int 
                                  int Nx_harmonics, int Ny_harmonics, int batch,
                                  rcwa_c128* C_dev);
 
+/* ----- structure rasterisation on the device ----- */
+
+/* The reference rasterises its structure on the host before ConvMat
+ * (main.cpp:48-91: a circle sampled at cell centres into er[layer][iy][ix],
+ * rcwa.hpp:56).  These shapes are sampled the same way on the device:
+ * x = (ix + 0.5) Lx / Nx, y = (iy + 0.5) Ly / Ny, and a sample is inside
+ *   RCWA_SHAPE_ELLIPSE:  ((x-cx)/ax)^2 + ((y-cy)/ay)^2 <= 1
+ *   RCWA_SHAPE_RECT:     |x-cx| <= ax and |y-cy| <= ay  (ay = inf: a line)
+ *   RCWA_SHAPE_CIRCLE:   (x-cx)^2 + (y-cy)^2 <= ax^2
+ * evaluated in FP64 without fused multiply-adds, so a grid matches a host
+ * rasterisation written with the same expressions bit for bit.  No
+ * periodic wrap: a shape across the cell edge is given once per image. */
+typedef enum { RCWA_SHAPE_ELLIPSE = 0, RCWA_SHAPE_RECT = 1, RCWA_SHAPE_CIRCLE = 2 } rcwa_shape_kind;
+
+typedef struct {
+  int layer, kind;
+  double cx, cy, ax, ay;
+  rcwa_c128 value; /* permittivity (or permeability) inside the shape */
+} rcwa_shape;
+
+/* Fills grid_dev (device, num_layers x Ny x Nx, [layer][iy][ix]) with
+ * background[layer] (host array) and then the shapes (host array) of each
+ * layer in their order, later shapes painting over earlier ones.  Returns
+ * when the grid is complete; grid_dev can then be passed to rcwa_solve* with
+ * grids_on_device = 1 (the step before ConvMat, without the host-to-device
+ * copy of the grids). */
+rcwa_status rcwa_rasterize(rcwa_ctx* ctx, int Nx, int Ny, double Lx, double Ly, int num_layers,
+                           const rcwa_c128* background, const rcwa_shape* shapes, int nshapes,
+                           rcwa_c128* grid_dev);
+
 /* ----- serving form (continuous batching) ----- */
 
 typedef struct rcwa_server rcwa_server;

@@ -2,6 +2,7 @@
 // takes the stream it enqueues on and returns 0 or a hipError_t value.
 #pragma once
 #include "common.hpp"
+#include "rcwa_amd.h"
 
 // zgemm.hip ------------------------------------------------------------------
 int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
@@ -59,6 +60,10 @@ int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, in
 int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch);
 
 // rcwa_kernels.hip -----------------------------------------------------------
+// Cell-centre rasterisation of layer-sorted shapes (shapes first[l] ..
+// first[l+1]-1 belong to layer l) over background bg[l]: grid [l][iy][ix].
+int rasterize(hipStream_t s, int Nx, int Ny, double Lx, double Ly, int L, const zc* bg,
+              const rcwa_shape* shapes, const int* first, zc* grid);
 // Truncated 2-D DFT of grids [b][iy][ix] at frequencies |fx| <= 2hx,
 // |fy| <= 2hy, normalised by 1/(Nx Ny): F[b][fy+2hy][fx+2hx].
 int convmat_dft(hipStream_t s, const zc* grids, int Nx, int Ny, int hx, int hy, zc* F,

@@ -2,8 +2,10 @@
 // convolution-matrix assembly (rcwa.cpp:6-82), P/Q assembly
 // (rcwa.cpp:264-285), mode constants (rcwa.cpp:317, 339) and the A/B
 // matrices (rcwa.cpp:335-338).
+#include <algorithm>
 #include "common.hpp"
 #include "kernels.hpp"
+#include "rcwa_amd.h"
 
 namespace {
 
@@ -183,6 +185,52 @@ inline dim3 grid2(long long elems, int batch) {
 size_t convmat_rowbuf_elems(int Nx, int Ny, int hx, int batch) {
   (void)Nx;
   return (size_t)batch * Ny * (4 * hx + 1);
+}
+
+namespace {
+
+// Cell-centre sampling of the shapes of each layer (first[l] .. first[l+1]-1
+// in the layer-sorted list, painter's order).  Contraction is off so that
+// every inside test rounds as the plain expression does.
+__global__ void rasterize_kernel(int Nx, int Ny, double Lx, double Ly, int L, const zc* bg,
+                                 const rcwa_shape* shapes, const int* first, zc* grid) {
+#pragma clang fp contract(off)
+  const long long total = (long long)L * Ny * Nx;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int ix = (int)(idx % Nx);
+    const long long r = idx / Nx;
+    const int iy = (int)(r % Ny), l = (int)(r / Ny);
+    const double x = ((double)ix + 0.5) * Lx / Nx;
+    const double y = ((double)iy + 0.5) * Ly / Ny;
+    zc v = bg[l];
+    for (int k = first[l]; k < first[l + 1]; ++k) {
+      const rcwa_shape sh = shapes[k];
+      bool in;
+      if (sh.kind == RCWA_SHAPE_ELLIPSE) {
+        const double u = (x - sh.cx) / sh.ax, w = (y - sh.cy) / sh.ay;
+        in = u * u + w * w <= 1.0;
+      } else if (sh.kind == RCWA_SHAPE_RECT) {
+        in = fabs(x - sh.cx) <= sh.ax && fabs(y - sh.cy) <= sh.ay;
+      } else {
+        const double u = x - sh.cx, w = y - sh.cy;
+        in = u * u + w * w <= sh.ax * sh.ax;
+      }
+      if (in) v = zmake(sh.value.re, sh.value.im);
+    }
+    grid[idx] = v;
+  }
+}
+
+}  // namespace
+
+int rasterize(hipStream_t s, int Nx, int Ny, double Lx, double Ly, int L, const zc* bg,
+              const rcwa_shape* shapes, const int* first, zc* grid) {
+  const long long total = (long long)L * Ny * Nx;
+  const long long blocks = std::min<long long>((total + 255) / 256, 65536);
+  hipLaunchKernelGGL(rasterize_kernel, dim3((unsigned)blocks), dim3(256), 0, s, Nx, Ny, Lx, Ly, L,
+                     bg, shapes, first, grid);
+  return (int)hipGetLastError();
 }
 
 int convmat_dft(hipStream_t s, const zc* grids, int Nx, int Ny, int hx, int hy, zc* F, int batch,

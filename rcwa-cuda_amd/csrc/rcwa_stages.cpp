@@ -105,6 +105,43 @@ rcwa_status rcwa_convmat_batched(rcwa_ctx* ctx, const rcwa_c128* grids_dev, int 
   return RCWA_OK;
 }
 
+rcwa_status rcwa_rasterize(rcwa_ctx* ctx, int Nx, int Ny, double Lx, double Ly, int num_layers,
+                           const rcwa_c128* background, const rcwa_shape* shapes, int nshapes,
+                           rcwa_c128* grid_dev) {
+  if (!ctx || !background || !grid_dev || Nx <= 0 || Ny <= 0 || num_layers <= 0 || nshapes < 0 ||
+      (nshapes > 0 && !shapes) || !(Lx > 0) || !(Ly > 0))
+    return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  for (int k = 0; k < nshapes; ++k) {
+    const rcwa_shape& sh = shapes[k];
+    if (sh.layer < 0 || sh.layer >= num_layers || sh.kind < RCWA_SHAPE_ELLIPSE ||
+        sh.kind > RCWA_SHAPE_CIRCLE)
+      return rcwa_set_error(RCWA_E_ARG, "shape with a bad layer or kind");
+  }
+  Ctx& c = *C(ctx);
+  CKH(hipSetDevice(c.device));
+  // layer-sorted copy (stable: painter's order within a layer kept)
+  std::vector<rcwa_shape> sorted(shapes, shapes + nshapes);
+  std::stable_sort(sorted.begin(), sorted.end(),
+                   [](const rcwa_shape& a, const rcwa_shape& b) { return a.layer < b.layer; });
+  std::vector<int> first(num_layers + 1, 0);
+  for (const rcwa_shape& sh : sorted) ++first[sh.layer + 1];
+  for (int l = 0; l < num_layers; ++l) first[l + 1] += first[l];
+  const size_t nb = (size_t)num_layers * sizeof(zc), ns = sorted.size() * sizeof(rcwa_shape),
+               nf = first.size() * sizeof(int);
+  rcwa_status st = c.reserve(nb + ns + nf + 3 * 256);
+  if (st) return st;
+  zc* bg = (zc*)c.take(nb);
+  rcwa_shape* sd = (rcwa_shape*)c.take(ns > 0 ? ns : 1);
+  int* fd = (int*)c.take(nf);
+  if ((st = c.arena_check()) != RCWA_OK) return st;
+  CKH(hipMemcpyAsync(bg, background, nb, hipMemcpyHostToDevice, c.stream));
+  if (ns) CKH(hipMemcpyAsync(sd, sorted.data(), ns, hipMemcpyHostToDevice, c.stream));
+  CKH(hipMemcpyAsync(fd, first.data(), nf, hipMemcpyHostToDevice, c.stream));
+  CK(rasterize(c.stream, Nx, Ny, Lx, Ly, num_layers, bg, sd, fd, (zc*)grid_dev));
+  CKH(hipStreamSynchronize(c.stream));
+  return RCWA_OK;
+}
+
 rcwa_status rcwa_wave_vectors(double Lx, double Ly, const rcwa_source* src,
                               const rcwa_params* p, rcwa_c128* k_inc, rcwa_c128* Kx,
                               rcwa_c128* Ky, rcwa_c128* Kz_ref, rcwa_c128* Kz_trn) {

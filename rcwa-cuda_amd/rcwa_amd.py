@@ -55,6 +55,62 @@ class Results(ctypes.Structure):
                 ("r", ctypes.c_void_p), ("t", ctypes.c_void_p)]
 
 
+SHAPE_ELLIPSE, SHAPE_RECT, SHAPE_CIRCLE = 0, 1, 2
+
+
+class Shape(ctypes.Structure):
+    _fields_ = [("layer", ctypes.c_int), ("kind", ctypes.c_int), ("cx", ctypes.c_double),
+                ("cy", ctypes.c_double), ("ax", ctypes.c_double), ("ay", ctypes.c_double),
+                ("value", c128)]
+
+
+class DeviceGrid:
+    """Device memory of rcwa_rasterize's output: num_layers x Ny x Nx
+    complex128 ([layer][iy][ix]); .ptr for device_grids in Context.solve."""
+
+    def __init__(self, shape):
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 16
+        self.ptr = _hip_malloc(self.nbytes)
+
+    def download(self):
+        out = np.empty(self.shape, np.complex128)
+        rc = _hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.ptr),
+                              ctypes.c_size_t(self.nbytes), 2)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+_HIP = None
+
+
+def _hip():
+    global _HIP
+    if _HIP is None:
+        _HIP = ctypes.CDLL("libamdhip64.so")
+    return _HIP
+
+
+def _hip_malloc(nbytes):
+    p = ctypes.c_void_p()
+    rc = _hip().hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes))
+    if rc != 0:
+        raise RuntimeError("hipMalloc failed: %d" % rc)
+    return p.value
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("device", ctypes.POINTER(Device)), ("params", ctypes.POINTER(Params)),
                 ("src", ctypes.POINTER(Source)), ("nsrc", ctypes.c_int)]
@@ -88,7 +144,8 @@ EXPORTS = [
     "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
     "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
-    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_server_create", "rcwa_server_submit",
+    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_rasterize", "rcwa_server_create",
+    "rcwa_server_submit",
     "rcwa_server_poll", "rcwa_server_wait", "rcwa_server_cancel", "rcwa_server_clock",
     "rcwa_server_destroy",
 ]
@@ -238,6 +295,20 @@ class Context:
                 one += (ra.reshape(npol, 3, P * Q), ta.reshape(npol, 3, P * Q))
             out.append(one)
         return out
+
+    def rasterize(self, Nx, Ny, Lx, Ly, background, shapes):
+        """Shapes (tuples (layer, kind, cx, cy, ax, ay, value)) over
+        background[layer], sampled at cell centres on the device.  Returns a
+        DeviceGrid of shape (len(background), Ny, Nx)."""
+        bg = _cz(background)
+        L = bg.size
+        arr = (Shape * max(len(shapes), 1))(
+            *[Shape(int(l), int(k), float(cx), float(cy), float(ax), float(ay), _c(v))
+              for (l, k, cx, cy, ax, ay, v) in shapes])
+        g = DeviceGrid((L, Ny, Nx))
+        _check(lib().rcwa_rasterize(self.h, Nx, Ny, ctypes.c_double(Lx), ctypes.c_double(Ly), L,
+                                    _ptr(bg), arr, len(shapes), ctypes.c_void_p(g.ptr)))
+        return g
 
     def last_source_columns(self, count):
         buf = np.zeros(count, np.complex128)

@@ -46,6 +46,10 @@ class Problem:
                 self.er_trn, self.ur_trn)
 
 
+# shape kinds of rcwa_rasterize (include/rcwa_amd.h)
+SHAPE_ELLIPSE, SHAPE_RECT, SHAPE_CIRCLE = 0, 1, 2
+
+
 def _centres(n, L):
     return (np.arange(n) + 0.5) * L / n
 
@@ -93,18 +97,22 @@ def c2_materials(lams, nmat=5, seed=2):
     return out
 
 
+def c2_halfwidths(period=120.0, cd=40.0, height=100.0, nslice=20, sidewall_deg=85.0):
+    """Half-widths of the trapezoid line's slices (layer 0 on top)."""
+    dz = height / nslice
+    cot = 1.0 / math.tan(math.radians(sidewall_deg))
+    return [(cd + 2.0 * ((k + 0.5) * dz - height / 2) * cot) / 2 for k in range(nslice)]
+
+
 def c2_profile(grid=512, period=120.0, cd=40.0, height=100.0, nslice=20, sidewall_deg=85.0):
     """Trapezoid line, y-invariant, cut into nslice slabs (layer 0 on top).
     Returns a boolean mask [nslice][grid] along x."""
     x = _centres(grid, period)
-    dz = height / nslice
-    cot = 1.0 / math.tan(math.radians(sidewall_deg))
+    hw = c2_halfwidths(period, cd, height, nslice, sidewall_deg)
     masks = np.zeros((nslice, grid), bool)
     for k in range(nslice):
-        zc = (k + 0.5) * dz
-        w = cd + 2.0 * (zc - height / 2) * cot
-        masks[k] = np.abs(x - period / 2) <= w / 2
-    return masks, np.full(nslice, dz)
+        masks[k] = np.abs(x - period / 2) <= hw[k]
+    return masks, np.full(nslice, height / nslice)
 
 
 def c2_sweep(nlam=200, grid=512, h=10, nslice=20, lam_lo=250.0, lam_hi=800.0, seed=2,
@@ -127,8 +135,10 @@ def c2_sweep(nlam=200, grid=512, h=10, nslice=20, lam_lo=250.0, lam_hi=800.0, se
     return probs
 
 
-def random_shapes_layer(rng, grid, Lx, Ly):
-    """1-3 ellipses/rectangles with eps Re in [1,16], Im in [0,2]."""
+def random_shapes_layer(rng, grid, Lx, Ly, shapes=None, layer=0):
+    """1-3 ellipses/rectangles with eps Re in [1,16], Im in [0,2].  If shapes
+    is a list, the layer's (layer, kind, cx, cy, ax, ay, eps) entries are
+    appended to it and its background is returned with the grid."""
     x = _centres(grid, Lx)[None, :]
     y = _centres(grid, Ly)[:, None]
     eb = rng.uniform(1, 16) + 1j * rng.uniform(0, 2)
@@ -139,10 +149,14 @@ def random_shapes_layer(rng, grid, Lx, Ly):
         es = rng.uniform(1, 16) + 1j * rng.uniform(0, 2)
         if rng.integers(0, 2) == 0:
             inside = ((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2 <= 1.0
+            kind = SHAPE_ELLIPSE
         else:
             inside = (np.abs(x - cx) <= ax) & (np.abs(y - cy) <= ay)
+            kind = SHAPE_RECT
         g = np.where(inside, es, g)
-    return g
+        if shapes is not None:
+            shapes.append((layer, kind, cx, cy, ax, ay, es))
+    return g if shapes is None else (g, eb)
 
 
 def min_abs_kz(hx, hy, Lx, Ly, lam0, theta, phi, media):
@@ -174,6 +188,34 @@ def c4_stream(count=2000, seed=4, grid=256, rate_hz=2000.0, hs=(3, 5, 7, 10, 12)
         out.append((t, Problem(er, np.ones_like(er), thick, period, period, lam, th, 0.0,
                                [(1.0, 0.0)], h, h, 1.0, 1.0, 2.25, 1.0, name="c4")))
     return out
+
+
+def c1_shapes():
+    """The C1 hole as a shape list: (Lx, Ly, background[1], shapes)."""
+    return 500.0, 500.0, [15.0 + 0.1j], [(0, SHAPE_CIRCLE, 500.0 / 2, 500.0 / 2, 150.0, 0.0, 1.0)]
+
+
+def c2_shapes(lam_index, nlam=200, nslice=20, lam_lo=250.0, lam_hi=800.0, seed=2):
+    """The C2 line at one wavelength of the sweep as a shape list: each slice
+    a line (rectangle of infinite height) of its material over air."""
+    mats = c2_materials(np.linspace(lam_lo, lam_hi, nlam), 5, seed)
+    hw = c2_halfwidths(nslice=nslice)
+    shapes = [(k, SHAPE_RECT, 120.0 / 2, 0.0, hw[k], math.inf, mats[(k * 5) // nslice, lam_index])
+              for k in range(nslice)]
+    return 120.0, 120.0, [1.0] * nslice, shapes
+
+
+def c5_shapes(nlayer=50):
+    """The C5 holes as a shape list: per layer the centre hole and the four
+    corner images, radius 60 -> 45 nm, over the layer's eps."""
+    Lx, Ly = 150.0, 260.0
+    bg, shapes = [], []
+    for k in range(nlayer):
+        r = 60.0 - 15.0 * k / (nlayer - 1)
+        bg.append(2.1 if k % 2 == 0 else 4.0 + 0.05j)
+        for cx, cy in [(Lx / 2, Ly / 2), (0.0, 0.0), (0.0, Ly), (Lx, 0.0), (Lx, Ly)]:
+            shapes.append((k, SHAPE_CIRCLE, cx, cy, r, 0.0, 1.0))
+    return Lx, Ly, bg, shapes
 
 
 def c5_large(grid=1024, h=20, nlayer=50):
