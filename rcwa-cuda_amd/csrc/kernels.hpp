@@ -80,9 +80,13 @@ int build_pq(hipStream_t s, int N, ZCMat erc, ZCMat urc, ZCMat erci, ZCMat urci,
 // lam = usqrt(lam2), ilam = 1/lam, X = exp(-k0 t lam) (rcwa.cpp:317, 339).
 int layer_modes(hipStream_t s, int n, const zc* lam2, zc* lam, zc* ilam, zc* X,
                 const double* k0t, int batch);
-// A = Wi + Vi V0, B = Wi - Vi V0 for the 2x2-block-diagonal V0 of the gap
-// medium, given by its four diagonals v0[b][4][N] (rcwa.cpp:335-338).
-int build_ab(hipStream_t s, int N, ZCMat Wi, ZCMat Vi, const zc* v0, ZMat A, ZMat B, int batch);
+// out = M V0 for the gap medium's V0 given by its four diagonals
+// v0[b][4][N] (rcwa.cpp:225).
+int mul_v0(hipStream_t s, int N, ZCMat M, const zc* v0, ZMat out, int batch);
+// A, B of rcwa.cpp:335-338 from W^-1 and Y = W^-1 P V0: with V = Q W L^-1
+// and P Q = W L^2 W^-1, V^-1 V0 = L^-1 Y, so A = W^-1 + L^-1 Y and
+// B = W^-1 - L^-1 Y (L = diag(lam), ilam = 1/lam).  A overwrites Y.
+int build_ab(hipStream_t s, int n, ZCMat Wi, ZMat Y, const zc* ilam, ZMat B, int batch);
 // Dense n x n matrix from a 2x2 block structure of diagonals d + b*dstride, [4][N].
 int expand_block_diag(hipStream_t s, int N, const zc* d, long long dstride, ZMat M, int batch);
 // Gather columns cols[0..ncol) of M into out[b][c][n].

@@ -186,15 +186,19 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
     c.stats.n_eig_launches += 1;
     c.stats.n_eig_matrices += Bl;
   }
-  // lambda, X (rcwa.cpp:317, 339); V = Q W diag(1/lambda) (rcwa.cpp:326-330)
+  // lambda, X (rcwa.cpp:317, 339)
   CK(layer_modes(s, n, lam2, lam, ilam, X, k0t, Bl));
-  CK(zscale_cols_to(s, n, n, CM(b3), ilam, n, M(b0), Bl));
-  // FP64 in both forms as well: V carries the eigenvectors into W^-1 and V^-1
-  CK(zgemm_batched(s, n, n, n, one, CM(b1), CM(b0), zero, none, M(b2), Bl));
-  // W^-1 W0 = W^-1, V^-1 V0; A, B (rcwa.cpp:335-338)
+  // A, B = W^-1 W0 +- V^-1 V0 (rcwa.cpp:326-338, W0 = I).  The reference
+  // forms V = Q W diag(1/lambda) and inverts it; since P Q = W diag(lambda^2)
+  // W^-1, V^-1 = diag(1/lambda) W^-1 P, so V^-1 V0 = diag(1/lambda) W^-1
+  // (P V0): one product instead of a product and an inverse.  P V0 (V0 is
+  // a 2x2 block of diagonals) -> b2; W^-1 in place of W; Y = W^-1 P V0 ->
+  // b0 (P is not needed after this), FP64 in both precision forms (it
+  // carries the eigenvectors); A -> b0, B -> b1.
+  CK(mul_v0(s, N, CM(b0), v0, M(b2), Bl));
   CK(gj_inverse_batched(s, n, M(b3), Bl, w.gj, info_sing));
-  CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));
-  CK(build_ab(s, N, CM(b3), CM(b2), v0, M(b0), M(b1), Bl));
+  CK(zgemm_batched(s, n, n, n, one, CM(b3), CM(b2), zero, none, M(b0), Bl));
+  CK(build_ab(s, n, CM(b3), M(b0), ilam, M(b1), Bl));
   // A^-1
   CK(zcopy_batched(s, n, n, CM(b0), M(b2), Bl));
   CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));

@@ -132,21 +132,37 @@ __global__ void modes_kernel(int n, const zc* lam2, zc* lam, zc* ilam, zc* X, co
   }
 }
 
-__global__ void build_ab_kernel(int N, ZCMat Wi, ZCMat Vi, const zc* v0, ZMat A, ZMat B) {
+// out = M V0 for the gap medium's V0, given by its four N-diagonals
+// v0[b][(2 bi + bj) N + m] (block (bi, bj), rcwa.cpp:225): column j = bj N + m
+// of the product is M(:, m) V0_{0,bj}(m) + M(:, N + m) V0_{1,bj}(m).
+__global__ void mul_v0_kernel(int N, ZCMat M, const zc* v0, ZMat out) {
   const int b = blockIdx.y, n = 2 * N;
-  const zc* w = Wi.p + (long long)b * Wi.stride;
-  const zc* v = Vi.p + (long long)b * Vi.stride;
+  const zc* m = M.p + (long long)b * M.stride;
   const zc* d = v0 + (long long)b * 4 * N;
-  zc* Ab = A.p + (long long)b * A.stride;
-  zc* Bb = B.p + (long long)b * B.stride;
+  zc* o = out.p + (long long)b * out.stride;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)n * n;
        idx += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(idx % n), j = (int)(idx / n);
     const int bj = j / N, jj = j - bj * N;
-    const zc t = zadd(zmul(v[i + (long long)jj * Vi.ld], d[(0 * 2 + bj) * N + jj]),
-                      zmul(v[i + (long long)(N + jj) * Vi.ld], d[(1 * 2 + bj) * N + jj]));
+    o[i + (long long)j * out.ld] = zadd(zmul(m[i + (long long)jj * M.ld], d[(0 * 2 + bj) * N + jj]),
+                                        zmul(m[i + (long long)(N + jj) * M.ld],
+                                             d[(1 * 2 + bj) * N + jj]));
+  }
+}
+
+// A = Wi + diag(ilam) Y and B = Wi - diag(ilam) Y; A overwrites Y.
+__global__ void build_ab_kernel(int n, ZCMat Wi, ZMat Y, const zc* ilam, ZMat B) {
+  const int b = blockIdx.y;
+  const zc* w = Wi.p + (long long)b * Wi.stride;
+  zc* y = Y.p + (long long)b * Y.stride;
+  zc* Bb = B.p + (long long)b * B.stride;
+  const zc* il = ilam + (long long)b * n;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)n * n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % n), j = (int)(idx / n);
+    const zc t = zmul(il[i], y[i + (long long)j * Y.ld]);
     const zc wv = w[i + (long long)j * Wi.ld];
-    Ab[i + (long long)j * A.ld] = zadd(wv, t);
+    y[i + (long long)j * Y.ld] = zadd(wv, t);
     Bb[i + (long long)j * B.ld] = zsub(wv, t);
   }
 }
@@ -264,9 +280,14 @@ int layer_modes(hipStream_t s, int n, const zc* lam2, zc* lam, zc* ilam, zc* X, 
   return (int)hipGetLastError();
 }
 
-int build_ab(hipStream_t s, int N, ZCMat Wi, ZCMat Vi, const zc* v0, ZMat A, ZMat B, int batch) {
-  hipLaunchKernelGGL(build_ab_kernel, grid2(4LL * N * N, batch), dim3(256), 0, s, N, Wi, Vi, v0,
-                     A, B);
+int mul_v0(hipStream_t s, int N, ZCMat M, const zc* v0, ZMat out, int batch) {
+  hipLaunchKernelGGL(mul_v0_kernel, grid2(4LL * N * N, batch), dim3(256), 0, s, N, M, v0, out);
+  return (int)hipGetLastError();
+}
+
+int build_ab(hipStream_t s, int n, ZCMat Wi, ZMat Y, const zc* ilam, ZMat B, int batch) {
+  hipLaunchKernelGGL(build_ab_kernel, grid2((long long)n * n, batch), dim3(256), 0, s, n, Wi, Y,
+                     ilam, B);
   return (int)hipGetLastError();
 }
 
