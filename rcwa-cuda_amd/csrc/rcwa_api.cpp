@@ -202,15 +202,16 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // A^-1
   CK(zcopy_batched(s, n, n, CM(b0), M(b2), Bl));
   CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));
-  // M1 = X B A^-1 X
+  // B A^-1 -> b3; C2 = A - (B A^-1) B -> b2 (rcwa.cpp:348, the reference's
+  // B (A^-1 B) reassociated so that B A^-1 serves twice; A^-1 is not needed
+  // after this)
   CK(c.gemm(n, n, n, one, CM(b1), CM(b2), zero, none, M(b3), Bl));
+  CK(c.gemm(n, n, n, mone, CM(b3), CM(b1), one, CM(b0), M(b2), Bl));
+  // M1 = X B A^-1 X (in place)
   CK(zscale_rc(s, n, n, M(b3), X, n, X, n, Bl));
   // D = A - M1 B ; R11 = M1 A - B (rcwa.cpp:346-347)
   CK(c.gemm(n, n, n, mone, CM(b3), CM(b1), one, CM(b0), M(b4), Bl));
   CK(c.gemm(n, n, n, one, CM(b3), CM(b0), mone, CM(b1), M(b5), Bl));
-  // C2 = A - B A^-1 B (rcwa.cpp:348)
-  CK(c.gemm(n, n, n, one, CM(b2), CM(b1), zero, none, M(b3), Bl));
-  CK(c.gemm(n, n, n, mone, CM(b1), CM(b3), one, CM(b0), M(b2), Bl));
   // S11 = D^-1 R11 ; S12 = D^-1 X C2
   CK(gj_inverse_batched(s, n, M(b4), Bl, w.gj, info_sing));
   CK(c.gemm(n, n, n, one, CM(b4), CM(b5), zero, none, S11, Bl));
