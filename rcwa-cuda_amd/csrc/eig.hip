@@ -711,7 +711,7 @@ __global__ __launch_bounds__(NT) void qr_apply_kernel(int n, ZMat Tm, ZMat Zm, Q
   __shared__ zc ts[G * TC];
   const int b = blockIdx.y, tid = threadIdx.x;
   const int st = blockIdx.x * NT + tid;
-  const bool live = st < (ph == 0 ? 2 * n : n);
+  const bool live = st < n;
   zc* T = Tm.p + (long long)b * Tm.stride;
   zc* Z = Zm.p + (long long)b * Zm.stride;
   const QrLogK LK = qr_log_k(log, b);
@@ -1538,13 +1538,12 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
         hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 2>), dim3(batch), dim3(256), 0, s, n, T,
                            Z, info, maxit_per_row, log);
       if ((e = (int)hipGetLastError())) return e;
-      hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
-                         dim3((2 * n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n,
-                         T, Z, log, 0);
-      if ((e = (int)hipGetLastError())) return e;
+      // the log reaches the Schur vectors only: the sweeps keep the active
+      // block of T up to date, and eig_batched forms the whole Schur form
+      // afterwards as Z^H A Z
       hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
                          dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T,
-                         Z, log, 1);
+                         Z, log, 0);
       if ((e = (int)hipGetLastError())) return e;
       int pending = 0;
       if ((e = (int)hipMemcpyAsync(&pending, log.pending, sizeof(int), hipMemcpyDeviceToHost, s)))
@@ -1575,7 +1574,7 @@ int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch) {
 }
 
 size_t eig_workspace_bytes(int n, int batch) {
-  return 2 * (size_t)batch * n * n * sizeof(zc) + hess_blocked_ws_bytes(n, batch) +
+  return 3 * (size_t)batch * n * n * sizeof(zc) + hess_blocked_ws_bytes(n, batch) +
          schur_ws_bytes(n, batch) + 1024;
 }
 
@@ -1584,10 +1583,13 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   if (n <= 0 || batch <= 0) return 0;
   zc* Zp = (zc*)ws;
   zc* Xp = Zp + (size_t)batch * n * n;
-  ZMat Z{Zp, n, (long long)n * n}, X{Xp, n, (long long)n * n};
-  void* hws = (void*)(((uintptr_t)(Xp + (size_t)batch * n * n) + 255) & ~(uintptr_t)255);
+  zc* Ap = Xp + (size_t)batch * n * n;  // a copy of A, later Z^H
+  const long long nn = (long long)n * n;
+  ZMat Z{Zp, n, nn}, X{Xp, n, nn};
+  void* hws = (void*)(((uintptr_t)(Ap + (size_t)batch * n * n) + 255) & ~(uintptr_t)255);
   int e;
   if (marks) (void)hipEventRecord(marks[0], s);
+  if ((e = zcopy_batched(s, n, n, ZCMat{A.p, A.ld, A.stride}, ZMat{Ap, n, nn}, batch))) return e;
   // RCWA_HESS=unblocked selects the one-workgroup unblocked reduction.
   static const bool unblocked = [] {
     const char* v = getenv("RCWA_HESS");
@@ -1601,6 +1603,17 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   if (marks) (void)hipEventRecord(marks[1], s);
   void* sws = (void*)(((uintptr_t)hws + hess_blocked_ws_bytes(n, batch) + 255) & ~(uintptr_t)255);
   if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws, log_cap))) return e;
+  // The Schur form T = Z^H A Z on the matrix cores (the sweeps left the
+  // entries of T outside each active block stale): X <- A Z, Ap <- Z^H,
+  // A <- Z^H (A Z).
+  const zc one = zmake(1, 0), zero = zmake(0, 0);
+  if ((e = zgemm_batched(s, n, n, n, one, ZCMat{Ap, n, nn}, ZCMat{Zp, n, nn}, zero,
+                         ZCMat{nullptr, 0, 0}, X, batch)))
+    return e;
+  if ((e = zconj_transpose_batched(s, n, n, ZCMat{Zp, n, nn}, ZMat{Ap, n, nn}, batch))) return e;
+  if ((e = zgemm_batched(s, n, n, n, one, ZCMat{Ap, n, nn}, ZCMat{Xp, n, nn}, zero,
+                         ZCMat{nullptr, 0, 0}, A, batch)))
+    return e;
   if (marks) (void)hipEventRecord(marks[2], s);
   if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch))) return e;
   hipLaunchKernelGGL(diag_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n,

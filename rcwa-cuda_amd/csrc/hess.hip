@@ -211,6 +211,13 @@ inline dim3 g2(long long elems, int batch) {
 
 }  // namespace
 
+int zconj_transpose_batched(hipStream_t s, int rows, int cols, ZCMat src, ZMat dst, int batch) {
+  if (rows <= 0 || cols <= 0 || batch <= 0) return 0;
+  hipLaunchKernelGGL(conj_transpose_kernel, g2((long long)rows * cols, batch), dim3(256), 0, s,
+                     rows, cols, src.p, src.ld, src.stride, dst.p, dst.stride);
+  return (int)hipGetLastError();
+}
+
 size_t hess_blocked_ws_bytes(int n, int batch) {
   // V, Y (n x NB), T (NB x NB), VH (NB x n), W (NB x n), W2 (NB x n), QV (n x NB), QVT (n x NB)
   const size_t per = (size_t)n * NB * 7 + (size_t)NB * NB * 2;

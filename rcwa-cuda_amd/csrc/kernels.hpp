@@ -18,6 +18,8 @@ int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* i
 
 // Elementwise helpers.
 int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag);
+// dst (cols x rows, leading dimension cols) = src^H (rows x cols).
+int zconj_transpose_batched(hipStream_t s, int rows, int cols, ZCMat src, ZMat dst, int batch);
 int zcopy_batched(hipStream_t s, int rows, int cols, ZCMat src, ZMat dst, int batch);
 // M(i,j) = r[b*rstride + i] * M(i,j) * c[b*cstride + j]; r or c may be null.
 int zscale_rc(hipStream_t s, int rows, int cols, ZMat M, const zc* r, long long rstride,
