@@ -9,9 +9,10 @@ The JSON line reports:
   value        solves/s (one solve = one wavelength with both polarisations)
   latency_ms   p50/p99 of the per-step wall time (every solve of a step
                completes with the step)
-  roofline     the QR-sweep kernel of the eigensolver (the dominant kernel),
-               its algorithmic FP64 flops per launch over its event-timed
-               average duration, against the FP64 matrix peak
+  roofline     the QR phase of the eigensolver (sweep kernel, log apply to
+               the Schur vectors, the two products forming the Schur form;
+               the dominant phase), its algorithmic FP64 flops per launch
+               over its event-timed average duration, against the FP64 peak
   cpu_baseline the CPU FP64 oracle on a bounded sample (one layer S-matrix
                and one Redheffer product of the same workload, one thread
                each), extrapolated to a full solve
@@ -34,10 +35,10 @@ sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
 
 METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
 # HBM bytes (read + written) of the QR phase per 882x882 layer matrix, from
-# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR kernels at 240
-# matrices per launch, scaled by the counters' reading on a copy of known size
-# (profiles/qr_phase_21x21_pmc_traffic.md).
-QR_PHASE_HBM_BYTES_PER_MATRIX = 3492.8e9 / 240
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR phase's kernels at
+# 240 matrices per launch, scaled by the counters' reading on a copy of known
+# size (profiles/qr_phase_21x21_pmc_traffic.md).
+QR_PHASE_HBM_BYTES_PER_MATRIX = 3063.2e9 / 240
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -269,9 +270,10 @@ def main():
     per_launch_mats = st["n_eig_matrices"] / launches
     avg_ms = st["ms_eig_schur"] / launches
     achieved = f_qr * per_launch_mats / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
-    roofline = {"kernel": "QR phase: schur_pipe_kernel + 2 x qr_apply_kernel (complex "
-                          "single-shift QR sweeps, two chained per pass; Schur form and "
-                          "Schur vectors)",
+    roofline = {"kernel": "QR phase: schur_pipe_kernel + qr_apply_kernel + 2 x zgemm_kernel "
+                          "(complex single-shift QR sweeps on the Hessenberg matrix, two "
+                          "chained per pass; their rotations applied to the Schur vectors Z; "
+                          "Schur form T = Z^H A Z)",
                 "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "traffic": QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats,

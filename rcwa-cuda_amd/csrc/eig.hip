@@ -554,38 +554,12 @@ __device__ __forceinline__ SweepGroup<G> load_group(const LogView& L, int g0, in
   return g;
 }
 
-// Strip st of a deferred pass.  Phase 0: st < n is row st of Z, st >= n row
-// st - n of T (both take column rotations); phase 1: column st of T (row
-// rotations).  Returns the sweeps of the group that apply to it as a mask.
-template <int G>
-__device__ __forceinline__ unsigned strip_of(int ph, int st, int n, const SweepGroup<G>& g,
-                                             zc* T, long long ld, zc* Z, long long ldz,
-                                             zc*& base, long long& step) {
-  unsigned mask = 0;
-  if (ph == 1) {
-    base = T + (long long)st * ld;
-    step = 1;
-#pragma unroll
-    for (int a = 0; a < G; ++a) mask |= (a < g.m && st > g.iu[a] ? 1u : 0u) << a;
-  } else if (st < n) {
-    base = Z + st;
-    step = ldz;
-    mask = (1u << g.m) - 1u;
-  } else {
-    base = T + (st - n);
-    step = ld;
-#pragma unroll
-    for (int a = 0; a < G; ++a) mask |= (a < g.m && st - n < g.il[a] ? 1u : 0u) << a;
-  }
-  return mask;
-}
-
 // LDS table of one chunk of time steps T0 .. T0+TC-1: entry a * TC + u holds
 // the rotation of sweep a of the group at time T0 + u, the identity where that
-// sweep has none, conjugated for row rotations (ph == 1).
+// sweep has none.
 template <int G, int TC>
-__device__ void fill_table(const SweepGroup<G>& g, const QrLogM& L, int T0, int ph, double* tc,
-                           zc* ts, int tid, int nt) {
+__device__ void fill_table(const SweepGroup<G>& g, const QrLogM& L, int T0, double* tc, zc* ts,
+                           int tid, int nt) {
   for (int e = tid; e < G * TC; e += nt) {
     const int a = e / TC, k = T0 + (e % TC) - 2 * a;
     double c = 1.0;
@@ -601,18 +575,16 @@ __device__ void fill_table(const SweepGroup<G>& g, const QrLogM& L, int T0, int 
     if (k >= il && k < iu) {
       c = L.c[off + (k - il)];
       sr = L.s[off + (k - il)];
-      if (ph == 1) sr = zconj(sr);
     }
     tc[e] = c;
     ts[e] = sr;
   }
 }
 
-// Applies a group of logged sweeps to one strip, entries base[c * step],
+// Applies a group of logged sweeps to one row of Z, entries base[c * step],
 // c = lo..hi, for the chunk T0 .. T1 of the group's time range, as column
-// rotations read from the LDS table (fill_table conjugates them for strips
-// that take row rotations).  Sweeps outside `mask` act as the identity
-// (MASKED: some do).  Rotation k of sweep a acts at time tau = k + 2a, so one
+// rotations read from the LDS table.  Rotation k of sweep a acts at time
+// tau = k + 2a, so one
 // pass applies the group in the order of the unwindowed algorithm while
 // holding only the entries in flight, in a circular register window (slot of
 // column c: c mod W) whose slot indices are static inside an unrolled block of
@@ -620,8 +592,8 @@ __device__ void fill_table(const SweepGroup<G>& g, const QrLogM& L, int T0, int 
 // rotation for a sweep meets the identity -- so that the window stays in fixed
 // registers.  Entries still in flight at the chunk end are stored back and
 // reloaded by the next chunk, so chunks compose exactly.
-template <int G, int TC, bool MASKED>
-__device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsigned mask, zc* base,
+template <int G, int TC>
+__device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, zc* base,
                                   long long step, const double* tc, const zc* ts) {
   constexpr int W = 2 * G;
   const int lo = g.lo, hi = g.hi, first = g.tmin - 2 * (G - 1);
@@ -669,12 +641,8 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
       w[(u + 1) & (W - 1)] = pre[u];
 #pragma unroll
       for (int a = 0; a < G; ++a) {
-        double c = tc[a * TC + u + (tb - T0)];
-        zc sr = ts[a * TC + u + (tb - T0)];
-        if (MASKED && !((mask >> a) & 1u)) {
-          c = 1.0;
-          sr = zmake(0, 0);
-        }
+        const double c = tc[a * TC + u + (tb - T0)];
+        const zc sr = ts[a * TC + u + (tb - T0)];
         zc& x = w[(u - 2 * a) & (W - 1)];
         zc& y = w[(u - 2 * a + 1) & (W - 1)];
         const zc xv = x, yv = y;
@@ -700,40 +668,28 @@ __device__ void apply_chunk_strip(const SweepGroup<G>& g, int T0, int T1, unsign
   }
 }
 
-// Applies the rotation log left by the sweep kernel: one launch per phase,
-// blockIdx.y = matrix, each thread one strip of phase `ph` (phase 0: 2n
-// strips, rows of Z then rows of T; phase 1: n strips, columns of T).  The
-// strips of a matrix are independent within a phase, so the threads never
-// synchronise.
+// Applies the rotation log left by the sweep kernel to the Schur vectors:
+// blockIdx.y = matrix, each thread one row of Z (the rows are independent,
+// so the threads synchronise only to stage the rotation tables).
 template <int G, int TC, int NT>
-__global__ __launch_bounds__(NT) void qr_apply_kernel(int n, ZMat Tm, ZMat Zm, QrLog log, int ph) {
+__global__ __launch_bounds__(NT) void qr_apply_kernel(int n, ZMat Zm, QrLog log) {
   __shared__ double tc[G * TC];
   __shared__ zc ts[G * TC];
   const int b = blockIdx.y, tid = threadIdx.x;
-  const int st = blockIdx.x * NT + tid;
-  const bool live = st < n;
-  zc* T = Tm.p + (long long)b * Tm.stride;
-  zc* Z = Zm.p + (long long)b * Zm.stride;
+  const int row = blockIdx.x * NT + tid;
+  zc* zrow = Zm.p + (long long)b * Zm.stride + row;  // entries zrow[c * ld]
   const QrLogK LK = qr_log_k(log, b);
   const QrLogM L = qr_log_of(log, b);
   const int nsw = ((KInt)log.nsw)[b];
   for (int g0 = 0; g0 < nsw; g0 += G) {
     const SweepGroup<G> g = load_group<G>(LK, g0, min(G, nsw - g0));
-    zc* base = nullptr;
-    long long step = 0;
-    unsigned mask = 0;
-    if (live) mask = strip_of<G>(ph, st, n, g, T, Tm.ld, Z, Zm.ld, base, step);
-    if (!__syncthreads_or(mask != 0)) continue;
     const int t0 = (g.tmin / (2 * G)) * (2 * G);
     for (int T0 = t0; T0 <= g.tmax; T0 += TC) {
       const int T1 = min(T0 + TC - 1, g.tmax);
       __syncthreads();
-      fill_table<G, TC>(g, L, T0, ph, tc, ts, tid, NT);
+      fill_table<G, TC>(g, L, T0, tc, ts, tid, NT);
       __syncthreads();
-      if (mask == (1u << g.m) - 1u)
-        apply_chunk_strip<G, TC, false>(g, T0, T1, mask, base, step, tc, ts);
-      else if (mask)
-        apply_chunk_strip<G, TC, true>(g, T0, T1, mask, base, step, tc, ts);
+      if (row < n) apply_chunk_strip<G, TC>(g, T0, T1, zrow, Zm.ld, tc, ts);
     }
   }
 }
@@ -798,9 +754,8 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
   __shared__ zc s_mu;
   const int b = blockIdx.x;
   zc* T = Tm.p + (long long)b * Tm.stride;
-  zc* Z = Zm.p + (long long)b * Zm.stride;
   const QrLogM L = qr_log_of(log, b);
-  const long long ld = Tm.ld, ldz = Zm.ld;
+  const long long ld = Tm.ld;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const double eps = DBL_EPSILON;
   const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
@@ -1542,8 +1497,8 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
       // block of T up to date, and eig_batched forms the whole Schur form
       // afterwards as Z^H A Z
       hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
-                         dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, T,
-                         Z, log, 0);
+                         dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, Z,
+                         log);
       if ((e = (int)hipGetLastError())) return e;
       int pending = 0;
       if ((e = (int)hipMemcpyAsync(&pending, log.pending, sizeof(int), hipMemcpyDeviceToHost, s)))
