@@ -8,10 +8,15 @@
 // eigenvector back-transformation and the rank-nb updates of the blocked
 // Gauss-Jordan inverse.
 //
-// A complex product is four real FP64 MFMAs per 16x16x4 step on the same
-// operand registers: each lane loads its complex element once and feeds the
-// real part to two MFMAs and the imaginary part to the other two
-// (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), so no deinterleaving is needed.
+// A complex product is three real FP64 MFMAs per 16x16x4 step (the 3M form
+// of the complex product, Higham, "Stability of a method for multiplying
+// complex matrices with three real matrix multiplications", SIAM J. Matrix
+// Anal. Appl. 13 (1992)): P1 += Ar Br, P2 += Ai Bi, P3 += (Ar + Ai)(Br + Bi),
+// then Cr = P1 - P2 and Ci = P3 - (P1 + P2).  Each lane loads its complex
+// element once and forms the sum of its parts in registers.  The method is
+// normwise stable: the error of Ci is bounded by a small multiple of
+// u k max|A| max|B| like that of the four-product form; only the componentwise
+// bound of a tiny imaginary part is lost, which no consumer on the path uses.
 //
 // Lane layout of v_mfma_f64_16x16x4f64 (measured on gfx950 with an
 // in-bounds probe): lane l supplies A[l%16][l/16] and B[l/16][l%16]; result
@@ -35,9 +40,10 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alpha, ZCMat A,
-                                                    ZCMat B, zc beta, ZCMat D, ZMat C,
-                                                    int tiles_n) {
+// Two workgroups per CU (their LDS fits twice): registers capped at 256 per
+// lane, which the three accumulator sets still fit without spilling.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void zgemm_kernel(
+    int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int tiles_n) {
   __shared__ zc As[2][BK][BM + PADA];
   __shared__ zc Bs[2][BK][BN + PADB];
   const int b = blockIdx.z;
@@ -51,13 +57,14 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alph
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, lk = lane >> 4;
 
-  v4d accR[2][2], accI[2][2];
+  v4d acc1[2][2], acc2[2][2], acc3[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      accR[i][j] = v4d{0, 0, 0, 0};
-      accI[i][j] = v4d{0, 0, 0, 0};
+      acc1[i][j] = v4d{0, 0, 0, 0};
+      acc2[i][j] = v4d{0, 0, 0, 0};
+      acc3[i][j] = v4d{0, 0, 0, 0};
     }
 
   zc ra[4], rb[4];
@@ -96,21 +103,24 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alph
       for (int i = 0; i < 2; ++i) a[i] = As[cur][kk + lk][wm * 32 + i * 16 + li];
 #pragma unroll
       for (int j = 0; j < 2; ++j) bb[j] = Bs[cur][kk + lk][wn * 32 + j * 16 + li];
-      // eight independent accumulators between two updates of the same one
+      double as[2], bs[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) as[i] = a[i].x + a[i].y;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bs[j] = bb[j].x + bb[j].y;
+      // twelve independent accumulators between two updates of the same one
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          accR[i][j] = mfma(a[i].x, bb[j].x, accR[i][j]);
-          accI[i][j] = mfma(a[i].x, bb[j].y, accI[i][j]);
-        }
+        for (int j = 0; j < 2; ++j) acc1[i][j] = mfma(a[i].x, bb[j].x, acc1[i][j]);
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          accR[i][j] = mfma(-a[i].y, bb[j].y, accR[i][j]);
-          accI[i][j] = mfma(a[i].y, bb[j].x, accI[i][j]);
-        }
+        for (int j = 0; j < 2; ++j) acc2[i][j] = mfma(a[i].y, bb[j].y, acc2[i][j]);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc3[i][j] = mfma(as[i], bs[j], acc3[i][j]);
     }
     if (t + 1 < nk) {
       sstore(cur ^ 1);
@@ -131,7 +141,8 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, zc alph
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * 32 + i * 16 + 4 * r + lk;  // D[4r + l/16][l%16]
         if (row >= M) continue;
-        zc v = zmul(alpha, zmake(accR[i][j][r], accI[i][j][r]));
+        const double p1 = acc1[i][j][r], p2 = acc2[i][j][r];
+        zc v = zmul(alpha, zmake(p1 - p2, acc3[i][j][r] - (p1 + p2)));
         if (use_d) v = zfma(beta, Dp[row + (long long)col * D.ld], v);
         Cp[row + (long long)col * C.ld] = v;
       }
