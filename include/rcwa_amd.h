@@ -249,6 +249,46 @@ rcwa_status rcwa_rasterize(rcwa_ctx* ctx, int Nx, int Ny, double Lx, double Ly, 
                            const rcwa_c128* background, const rcwa_shape* shapes, int nshapes,
                            rcwa_c128* grid_dev);
 
+/* ----- sweep planner ----- */
+
+/* main.cpp:126-141 recomputes every stage for each call, ConvMat of the
+ * grids included.  A sweep plan holds a structure whose layers are made of
+ * materials: material[layer][iy][ix] (host, values 0 .. num_materials-1)
+ * with the permittivity (and permeability) of each material given per sweep
+ * point.  ConvMat is linear in the grid, so the plan computes the
+ * convolution matrix of every (layer, material) mask once, at creation, and
+ * each point's [[eps]] and [[mu]] are the sums eps_m [[mask_m]]: no grid is
+ * built or transformed per point.  Points whose eps (and mu) rows are equal
+ * (an angle sweep over non-dispersive media) share one [[eps]]^-1 (and
+ * [[mu]]^-1); with mu = NULL every material has mu = 1, [[mu]] = I and its
+ * inverse is not computed.  rcwa_sweep_run splits the points into chunks of
+ * equal size that fit the context's HBM budget (and its chunk limit) and
+ * runs each chunk as one batched solve. */
+typedef struct rcwa_sweep rcwa_sweep;
+
+typedef struct {
+  int Nx, Ny, num_layers, num_materials;
+  double Lx, Ly;
+  const unsigned char* material; /* host, num_layers * Ny * Nx */
+  const double* t;               /* num_layers thicknesses */
+} rcwa_sweep_structure;
+
+/* The plan keeps a pointer to ctx (which must outlive it) and device memory
+ * for the mask convolution matrices: (layer, material) pairs x PQ^2
+ * complex128. */
+rcwa_status rcwa_sweep_create(rcwa_ctx* ctx, const rcwa_sweep_structure* structure,
+                              const rcwa_params* params, rcwa_sweep** out);
+/* npoints sweep points.  Point p has the nsrc sources src[p*nsrc ..
+ * p*nsrc+nsrc-1] (sources of equal incidence share one S-matrix, as in
+ * rcwa_solve), the permittivities eps[p*num_materials + m] and, unless mu is
+ * NULL, the permeabilities mu[p*num_materials + m].  out[p*nsrc + s] as in
+ * rcwa_solve.  Same results as rcwa_solve_batch on the rasterised grids,
+ * to the rounding of the convolution matrices. */
+rcwa_status rcwa_sweep_run(rcwa_sweep* sweep, int npoints, const rcwa_source* src, int nsrc,
+                           const rcwa_c128* eps, const rcwa_c128* mu, rcwa_precision precision,
+                           rcwa_results* out);
+rcwa_status rcwa_sweep_destroy(rcwa_sweep* sweep);
+
 /* ----- serving form (continuous batching) ----- */
 
 typedef struct rcwa_server rcwa_server;

@@ -93,3 +93,13 @@ int build_ab(hipStream_t s, int n, ZCMat Wi, ZMat Y, const zc* ilam, ZMat B, int
 int expand_block_diag(hipStream_t s, int N, const zc* d, long long dstride, ZMat M, int batch);
 // Gather columns cols[0..ncol) of M into out[b][c][n].
 int gather_cols(hipStream_t s, int n, ZCMat M, const int* cols, int ncol, zc* out, int batch);
+// Sweep plans: mask grids out[k] = (material[layer[k]] == mat[k]) of npairs
+// (layer, material) pairs; out[map[g L + l]] = sum_k coef[g M + mat[k]] C[k]
+// over layer l's pairs first[l] .. first[l+1]-1 (N x N matrices, G
+// structures); dst[dmap[k]] = src[smap[k]] for matrices of `elems` entries.
+int material_masks(hipStream_t s, int Nx, int Ny, const unsigned char* material,
+                   const int* layer, const int* mat, int npairs, zc* out);
+int mix_convmat(hipStream_t s, int N, int L, int M, const int* first, const int* mat,
+                const zc* C, const zc* coef, const int* map, zc* out, int G);
+int gather_mats(hipStream_t s, long long elems, const zc* src, const int* smap, zc* dst,
+                const int* dmap, int count);

@@ -148,7 +148,8 @@ size_t layer_work_bytes(int N, int Bl) {
 
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
-                        ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing) {
+                        ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
+                        ZCMat erci_pre, ZCMat urci_pre) {
   const int n = 2 * N;
   const long long n2 = (long long)n * n, N2 = (long long)N * N;
   hipStream_t s = c.stream;
@@ -157,14 +158,23 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
   const ZCMat none{nullptr, 0, 0};
   zc *b0 = w.b[0], *b1 = w.b[1], *b2 = w.b[2], *b3 = w.b[3], *b4 = w.b[4], *b5 = w.b[5];
-  // [[eps]]^-1 and [[mu]]^-1 (rcwa.cpp:267-281: two LU solves each)
-  CK(zcopy_batched(s, N, N, erc, ZMat{w.erci, N, N2}, Bl));
-  CK(zcopy_batched(s, N, N, urc, ZMat{w.urci, N, N2}, Bl));
-  CK(gj_inverse_batched(s, N, ZMat{w.erci, N, N2}, Bl, w.gj, info_sing));
-  CK(gj_inverse_batched(s, N, ZMat{w.urci, N, N2}, Bl, w.gj, info_sing));
+  // [[eps]]^-1 and [[mu]]^-1 (rcwa.cpp:267-281: two LU solves each), unless
+  // the caller formed them
+  ZCMat erci{w.erci, N, N2}, urci{w.urci, N, N2};
+  if (erci_pre.p) {
+    erci = erci_pre;
+  } else {
+    CK(zcopy_batched(s, N, N, erc, ZMat{w.erci, N, N2}, Bl));
+    CK(gj_inverse_batched(s, N, ZMat{w.erci, N, N2}, Bl, w.gj, info_sing));
+  }
+  if (urci_pre.p) {
+    urci = urci_pre;
+  } else {
+    CK(zcopy_batched(s, N, N, urc, ZMat{w.urci, N, N2}, Bl));
+    CK(gj_inverse_batched(s, N, ZMat{w.urci, N, N2}, Bl, w.gj, info_sing));
+  }
   // P, Q (rcwa.cpp:264-285); Omega^2 = P Q (rcwa.cpp:288)
-  CK(build_pq(s, N, erc, urc, ZCMat{w.erci, N, N2}, ZCMat{w.urci, N, N2}, Kx, Ky, M(b0), M(b1),
-              Bl));
+  CK(build_pq(s, N, erc, urc, erci, urci, Kx, Ky, M(b0), M(b1), Bl));
   // Omega^2 feeds the eigensolver, which amplifies its perturbations (P Q
   // cancels heavily), so it is formed in FP64 in both precision forms: a
   // split-precision P Q alone moves T by ~3e-6 on the 7x7-harmonic single
@@ -287,6 +297,13 @@ struct Job {
   rcwa_results* out;       // problem's output array
   rcwa_host::WaveVectors wv;
   std::vector<cplx> cols;  // S11, S21 columns at the source entries
+  // a point of a sweep plan: [[eps]], [[mu]] are mixed from the plan's mask
+  // matrices with the point's material values; jobs of equal `row` have
+  // equal eps and mu
+  const SweepPlan* plan = nullptr;
+  const rcwa_c128* eps = nullptr;
+  const rcwa_c128* mu = nullptr;  // nullptr: mu = 1 for every material
+  int row = 0;
 };
 
 rcwa_status check_device(const rcwa_device* d, const rcwa_params* p) {
@@ -325,6 +342,19 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
   return bytes;
 }
 
+// HBM bytes a chunk of G sweep points adds (M materials): the material
+// values, and with `share` >= 0 distinct rows the inverses formed per row
+// (a buffer for all layers and one for the rows, for eps and unless mu = 1
+// for mu).  Zero for a chunk of plain problems (M = 0).
+static size_t sweep_chunk_extra_bytes(int N, int L, int G, int M, int share, bool mu_unit) {
+  if (M <= 0) return 0;
+  const size_t N2 = (size_t)N * N, T1 = std::max((size_t)G * L, (size_t)std::max(G, 1));
+  const size_t rows = (size_t)std::max(share, 0);
+  size_t z = 2 * (size_t)G * M;
+  if (share >= 0) z += (mu_unit ? 1 : 2) * T1 * N2 + rows * L * N2;
+  return z * sizeof(zc) + 2 * (rows * L + T1) * sizeof(int) + 6 * 256;
+}
+
 // Computes the S-matrices and efficiencies of G jobs that share (hx, hy,
 // Nx, Ny).  Jobs may have different layer counts: they are sorted by layer
 // count (descending) and their layers are laid out layer-major, so that at
@@ -351,10 +381,32 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   const int nfx = 4 * hx + 1, nfy = 4 * hy + 1;
   hipStream_t s = c.stream;
 
+  // sweep points: the distinct (eps, mu) rows among the jobs; when some row
+  // repeats, [[eps]]^-1 (and [[mu]]^-1) are formed once per row and layer
+  const SweepPlan* plan = jobs[0]->plan;
+  const int Mm = plan ? plan->M : 0;
+  bool mu_unit = true;
+  std::vector<int> urow(G, 0), urep;  // job -> its row's index; row -> first job
+  if (plan) {
+    std::map<int, int> seen;
+    for (int g = 0; g < G; ++g) {
+      mu_unit = mu_unit && jobs[g]->mu == nullptr;
+      auto it = seen.find(jobs[g]->row);
+      if (it == seen.end()) {
+        it = seen.emplace(jobs[g]->row, (int)urep.size()).first;
+        urep.push_back(g);
+      }
+      urow[g] = it->second;
+    }
+  }
+  const int U = (int)urep.size();
+  const bool share = plan && U < G && Lmax > 0;
+  const size_t extra = sweep_chunk_extra_bytes(N, Lmax, G, Mm, share ? U : -1, mu_unit);
+
   int Bl = std::max(TL, 1);
-  while (Bl > 1 && chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) > c.budget)
+  while (Bl > 1 && chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra > c.budget)
     Bl = (Bl + 1) / 2;
-  rcwa_status st = c.reserve(chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl));
+  rcwa_status st = c.reserve(chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra);
   if (st) return st;
 
   // ---- host precompute (O(N) per job)
@@ -414,6 +466,19 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   void* gjf = c.take(gj_workspace_bytes(n, std::max(G, 1)));
   LayerWork w;
   w.carve(c, N, Bl);
+  zc *dcoef_e = nullptr, *dcoef_u = nullptr, *erci_all = nullptr, *urci_all = nullptr;
+  zc* uinv = nullptr;
+  int* dgmap = nullptr;
+  if (plan) {
+    dcoef_e = (zc*)c.take((size_t)G * Mm * sizeof(zc));
+    dcoef_u = (zc*)c.take((size_t)G * Mm * sizeof(zc));
+    if (share) {
+      erci_all = (zc*)c.take(T1 * N2 * sizeof(zc));
+      if (!mu_unit) urci_all = (zc*)c.take(T1 * N2 * sizeof(zc));
+      uinv = (zc*)c.take((size_t)U * Lmax * N2 * sizeof(zc));
+      dgmap = (int*)c.take(2 * ((size_t)U * Lmax + TL) * sizeof(int));
+    }
+  }
   CK(c.arena_check());
 
   // layer-major destination of job g's layer l
@@ -439,7 +504,63 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   CKH(hipEventRecord(c.ev[0], s));
   // ---- ConvMat for every layer (main.cpp:126-138)
   const size_t G2 = (size_t)Nx * Ny;
-  for (int g = 0; g < G; ++g) {
+  if (plan && TL > 0) {
+    // sweep points: [[eps]] = sum_m eps_m [[mask_m]] (ConvMat is linear in
+    // the grid), [[mu]] likewise or I
+    std::vector<zc> ce((size_t)G * Mm), cu((size_t)G * Mm);
+    for (int g = 0; g < G; ++g)
+      for (int m = 0; m < Mm; ++m) {
+        ce[(size_t)g * Mm + m] = zmake(jobs[g]->eps[m].re, jobs[g]->eps[m].im);
+        cu[(size_t)g * Mm + m] =
+            jobs[g]->mu ? zmake(jobs[g]->mu[m].re, jobs[g]->mu[m].im) : zmake(1, 0);
+      }
+    CKH(hipMemcpyAsync(dcoef_e, ce.data(), ce.size() * sizeof(zc), hipMemcpyHostToDevice, s));
+    CKH(hipMemcpyAsync(dcoef_u, cu.data(), cu.size() * sizeof(zc), hipMemcpyHostToDevice, s));
+    CK(mix_convmat(s, N, Lmax, Mm, plan->dfirst, plan->dmat, plan->C, dcoef_e, dmap, erc, G));
+    if (mu_unit)
+      CK(zset_identity(s, N, ZMat{urc, N, N2}, TL, zmake(1, 0)));
+    else
+      CK(mix_convmat(s, N, Lmax, Mm, plan->dfirst, plan->dmat, plan->C, dcoef_u, dmap, urc, G));
+    if (share) {
+      // rows' inverses: gather each row's first job's matrices, invert them
+      // (sub-batches of at most G, the size of the Gauss-Jordan workspace
+      // at 2N), scatter them to every job of the row
+      const int K1 = U * Lmax;
+      std::vector<int> maps(2 * ((size_t)K1 + TL));
+      int* sm1 = maps.data();
+      int* dm1 = sm1 + K1;
+      int* sm2 = dm1 + K1;
+      int* dm2 = sm2 + TL;
+      for (int u = 0; u < U; ++u)
+        for (int l = 0; l < Lmax; ++l) {
+          sm1[u * Lmax + l] = off[l] + urep[u];
+          dm1[u * Lmax + l] = u * Lmax + l;
+        }
+      int k = 0;
+      for (int l = 0; l < Lmax; ++l)
+        for (int g = 0; g < act[l]; ++g, ++k) {
+          sm2[k] = urow[g] * Lmax + l;
+          dm2[k] = off[l] + g;
+        }
+      CKH(hipMemcpyAsync(dgmap, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice,
+                         s));
+      const int* dsm1 = dgmap;
+      const int* ddm1 = dsm1 + K1;
+      const int* dsm2 = ddm1 + K1;
+      const int* ddm2 = dsm2 + TL;
+      auto row_inverses = [&](const zc* src, zc* dst) -> rcwa_status {
+        CK(gather_mats(s, N2, src, dsm1, uinv, ddm1, K1));
+        for (int b0 = 0; b0 < K1; b0 += G)
+          CK(gj_inverse_batched(s, N, ZMat{uinv + (size_t)b0 * N2, N, N2}, std::min(G, K1 - b0),
+                                gjf, info_sing));
+        CK(gather_mats(s, N2, uinv, dsm2, dst, ddm2, TL));
+        return RCWA_OK;
+      };
+      if ((st = row_inverses(erc, erci_all)) != RCWA_OK) return st;
+      if (!mu_unit && (st = row_inverses(urc, urci_all)) != RCWA_OK) return st;
+    }
+  }
+  for (int g = 0; g < G && !plan; ++g) {
     const rcwa_device* d = jobs[g]->dev;
     const int L = d->num_layers;
     if (L == 0) continue;
@@ -467,12 +588,18 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   const bool progress = getenv("RCWA_PROGRESS") != nullptr;
   for (int l0 = 0; l0 < TL; l0 += Bl) {
     const int b = std::min(Bl, TL - l0);
+    // sweep points: inverses formed per row above; [[mu]] = I is its own
+    const ZCMat none_m{nullptr, 0, 0};
+    const ZCMat erci_pre = erci_all ? ZCMat{erci_all + (size_t)l0 * N2, N, N2} : none_m;
+    const ZCMat urci_pre = (plan && mu_unit) ? ZCMat{urc + (size_t)l0 * N2, N, N2}
+                           : urci_all        ? ZCMat{urci_all + (size_t)l0 * N2, N, N2}
+                                             : none_m;
     st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
                      dKx + (size_t)l0 * N, dKy + (size_t)l0 * N, dv0 + (size_t)l0 * 4 * N,
                      dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
                      ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
                      ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
-                     info_eig + l0, info_sing + l0);
+                     info_eig + l0, info_sing + l0, erci_pre, urci_pre);
     if (st) return st;
     if (progress) {
       CKH(hipStreamSynchronize(s));
@@ -683,6 +810,88 @@ rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_prec
       if (st) return st;
       i += G;
     }
+  }
+  for (auto& j : jobs) c.last_cols.insert(c.last_cols.end(), j.cols.begin(), j.cols.end());
+  return RCWA_OK;
+}
+
+rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_source* src,
+                        int nsrc, const rcwa_c128* eps, const rcwa_c128* mu,
+                        rcwa_precision prec, rcwa_results* out) {
+  if (npoints < 0 || (npoints > 0 && (!src || !eps || !out)) || nsrc <= 0)
+    return rcwa_set_error(RCWA_E_ARG, "bad sweep arguments");
+  c.precision = prec;
+  c.last_cols.clear();
+  const int M = plan.M, L = plan.dev.num_layers;
+  // rows: points with bitwise equal eps (and mu) share one, numbered in
+  // order of first appearance
+  std::vector<int> row(npoints);
+  std::map<std::string, int> rows;
+  for (int p = 0; p < npoints; ++p) {
+    std::string key((const char*)(eps + (size_t)p * M), (size_t)M * sizeof(rcwa_c128));
+    if (mu) key.append((const char*)(mu + (size_t)p * M), (size_t)M * sizeof(rcwa_c128));
+    row[p] = rows.emplace(key, (int)rows.size()).first->second;
+  }
+  const bool any_share = (int)rows.size() < npoints;
+  std::vector<Job> jobs;
+  for (int p = 0; p < npoints; ++p) {
+    const rcwa_source* ps = src + (size_t)p * nsrc;
+    const size_t first = jobs.size();
+    for (int si = 0; si < nsrc; ++si) {
+      const rcwa_source& sr = ps[si];
+      if (!(sr.lambda0 > 0)) return rcwa_set_error(RCWA_E_ARG, "lambda0 must be > 0");
+      bool found = false;
+      for (size_t k = first; k < jobs.size(); ++k) {
+        Job& j = jobs[k];
+        if (j.inc.lambda0 == sr.lambda0 && j.inc.theta == sr.theta && j.inc.phi == sr.phi) {
+          j.srcs.push_back(si);
+          found = true;
+          break;
+        }
+      }
+      if (found) continue;
+      Job j;
+      j.dev = &plan.dev;
+      j.par = &plan.par;
+      j.inc = {sr.lambda0, sr.theta, sr.phi};
+      j.media = {C(plan.par.er_ref), C(plan.par.ur_ref), C(plan.par.er_trn), C(plan.par.ur_trn)};
+      j.srcs.push_back(si);
+      j.src = ps;
+      j.out = out + (size_t)p * nsrc;
+      j.plan = &plan;
+      j.eps = eps + (size_t)p * M;
+      j.mu = mu ? mu + (size_t)p * M : nullptr;
+      j.row = row[p];
+      jobs.push_back(j);
+    }
+  }
+  if (jobs.empty()) return RCWA_OK;
+  // equal rows next to each other, so that they land in one chunk
+  std::vector<Job*> v;
+  for (auto& j : jobs) v.push_back(&j);
+  std::stable_sort(v.begin(), v.end(), [](const Job* a, const Job* b) { return a->row < b->row; });
+  // the largest chunk that fits (with a layer sub-batch of at least
+  // min(its layers, kMinLayerBatch)), then chunks of equal size
+  const int hx = plan.par.Nx_harmonics, hy = plan.par.Ny_harmonics, N = plan.N;
+  size_t G = v.size();
+  if (c.max_chunk > 0) G = std::min(G, (size_t)c.max_chunk);
+  for (;;) {
+    const int TL = (int)G * L;
+    const size_t need =
+        chunk_bytes(N, TL, L, plan.dev.Nx, plan.dev.Ny, hx, hy, false, (int)G,
+                    std::max(1, std::min(TL, kMinLayerBatch))) +
+        sweep_chunk_extra_bytes(N, L, (int)G, M, any_share ? (int)G : -1, mu == nullptr);
+    if (need <= c.budget || G == 1) break;
+    G = (G + 1) / 2;
+  }
+  // sizes differ by at most one, so that no chunk runs a near-empty batch
+  const size_t nch = (v.size() + G - 1) / G, base = v.size() / nch, rem = v.size() % nch;
+  for (size_t k = 0, i = 0; k < nch; ++k) {
+    const size_t g = base + (k < rem ? 1 : 0);
+    std::vector<Job*> part(v.begin() + i, v.begin() + i + g);
+    rcwa_status st = run_chunk(c, part);
+    if (st) return st;
+    i += g;
   }
   for (auto& j : jobs) c.last_cols.insert(c.last_cols.end(), j.cols.begin(), j.cols.end());
   return RCWA_OK;

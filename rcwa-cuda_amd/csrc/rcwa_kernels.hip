@@ -301,3 +301,77 @@ int gather_cols(hipStream_t s, int n, ZCMat M, const int* cols, int ncol, zc* ou
                      cols, ncol, out);
   return (int)hipGetLastError();
 }
+
+namespace {
+
+// Mask grids of (layer, material) pairs: out[k][iy][ix] = 1 where
+// material[layer[k]][iy][ix] == mat[k], else 0.
+__global__ void material_mask_kernel(long long cells, const unsigned char* material,
+                                     const int* layer, const int* mat, zc* out) {
+  const int k = blockIdx.y;
+  const unsigned char* m = material + (long long)layer[k] * cells;
+  const unsigned char want = (unsigned char)mat[k];
+  zc* o = out + (long long)k * cells;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < cells;
+       idx += (long long)gridDim.x * blockDim.x)
+    o[idx] = zmake(m[idx] == want ? 1.0 : 0.0, 0.0);
+}
+
+// out[map[g * L + l]] = sum over the pairs k = first[l] .. first[l+1]-1 of
+// coef[g * M + mat[k]] * C[k]: blockIdx.y = layer, blockIdx.z = structure.
+__global__ void mix_kernel(long long N2, int L, int M, const int* first, const int* mat,
+                           const zc* C, const zc* coef, const int* map, zc* out) {
+  const int l = blockIdx.y, g = blockIdx.z;
+  const int k0 = first[l], k1 = first[l + 1];
+  const zc* cf = coef + (long long)g * M;
+  zc* o = out + (long long)map[(long long)g * L + l] * N2;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < N2;
+       idx += (long long)gridDim.x * blockDim.x) {
+    zc acc = zmake(0, 0);
+    for (int k = k0; k < k1; ++k) acc = zfma(cf[mat[k]], C[(long long)k * N2 + idx], acc);
+    o[idx] = acc;
+  }
+}
+
+__global__ void gather_mats_kernel(long long elems, const zc* src, const int* smap, zc* dst,
+                                   const int* dmap) {
+  const int k = blockIdx.y;
+  const zc* s = src + (long long)smap[k] * elems;
+  zc* d = dst + (long long)dmap[k] * elems;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < elems;
+       idx += (long long)gridDim.x * blockDim.x)
+    d[idx] = s[idx];
+}
+
+}  // namespace
+
+int material_masks(hipStream_t s, int Nx, int Ny, const unsigned char* material,
+                   const int* layer, const int* mat, int npairs, zc* out) {
+  if (npairs <= 0) return 0;
+  hipLaunchKernelGGL(material_mask_kernel, grid2((long long)Nx * Ny, npairs), dim3(256), 0, s,
+                     (long long)Nx * Ny, material, layer, mat, out);
+  return (int)hipGetLastError();
+}
+
+int mix_convmat(hipStream_t s, int N, int L, int M, const int* first, const int* mat,
+                const zc* C, const zc* coef, const int* map, zc* out, int G) {
+  if (L <= 0 || G <= 0) return 0;
+  if (L > 65535 || G > 65535) return (int)hipErrorInvalidValue;
+  const long long N2 = (long long)N * N;
+  dim3 grid = grid2(N2, L);
+  grid.z = (unsigned)G;
+  hipLaunchKernelGGL(mix_kernel, grid, dim3(256), 0, s, N2, L, M, first, mat, C, coef, map,
+                     out);
+  return (int)hipGetLastError();
+}
+
+int gather_mats(hipStream_t s, long long elems, const zc* src, const int* smap, zc* dst,
+                const int* dmap, int count) {
+  // at most 65535 matrices per launch (the grid's y extent)
+  for (int c0 = 0; c0 < count; c0 += 65535) {
+    hipLaunchKernelGGL(gather_mats_kernel, grid2(elems, std::min(65535, count - c0)), dim3(256),
+                       0, s, elems, src, smap + c0, dst, dmap + c0);
+    if (int e = (int)hipGetLastError()) return e;
+  }
+  return 0;
+}

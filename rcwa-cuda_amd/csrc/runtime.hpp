@@ -63,9 +63,13 @@ constexpr int kMinLayerBatch = 256;
 size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool host_grids,
                    int G, int Bl);
 
+// erci_pre / urci_pre: [[eps]]^-1 / [[mu]]^-1 already formed (p != nullptr),
+// else the stage inverts erc / urc itself.
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
-                        ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing);
+                        ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
+                        ZCMat erci_pre = ZCMat{nullptr, 0, 0},
+                        ZCMat urci_pre = ZCMat{nullptr, 0, 0});
 
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
                          void* gjws, int* info_sing, bool keep_slots);
@@ -77,3 +81,24 @@ struct ProblemRef {
 };
 
 rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_precision prec);
+
+// A sweep plan (rcwa_sweep_create): a structure of num_layers layers made of
+// M materials, with the convolution matrices C[k] (device, N x N each) of its
+// (layer, material) masks; layer l's pairs are k = first[l] .. first[l+1]-1,
+// of material mat[k].
+struct SweepPlan {
+  Ctx* ctx = nullptr;
+  rcwa_device dev{};  // Nx, Ny, num_layers, Lx, Ly, t; no grids
+  rcwa_params par{};
+  std::vector<double> t;
+  int M = 0, N = 0;
+  std::vector<int> first, mat;
+  zc* C = nullptr;  // device
+  int* dfirst = nullptr;
+  int* dmat = nullptr;
+};
+
+// Runs npoints points of a plan (rcwa_sweep_run's arguments).
+rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_source* src,
+                        int nsrc, const rcwa_c128* eps, const rcwa_c128* mu,
+                        rcwa_precision prec, rcwa_results* out);

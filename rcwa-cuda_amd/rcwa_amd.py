@@ -111,6 +111,12 @@ def _hip_malloc(nbytes):
     return p.value
 
 
+class SweepStructure(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("num_layers", ctypes.c_int),
+                ("num_materials", ctypes.c_int), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double),
+                ("material", ctypes.c_void_p), ("t", ctypes.c_void_p)]
+
+
 class Problem(ctypes.Structure):
     _fields_ = [("device", ctypes.POINTER(Device)), ("params", ctypes.POINTER(Params)),
                 ("src", ctypes.POINTER(Source)), ("nsrc", ctypes.c_int)]
@@ -144,7 +150,8 @@ EXPORTS = [
     "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
     "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
-    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_rasterize", "rcwa_server_create",
+    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_rasterize", "rcwa_sweep_create",
+    "rcwa_sweep_run", "rcwa_sweep_destroy", "rcwa_server_create",
     "rcwa_server_submit",
     "rcwa_server_poll", "rcwa_server_wait", "rcwa_server_cancel", "rcwa_server_clock",
     "rcwa_server_destroy",
@@ -310,6 +317,13 @@ class Context:
                                     _ptr(bg), arr, len(shapes), ctypes.c_void_p(g.ptr)))
         return g
 
+    def sweep(self, material, thick, Lx, Ly, num_materials, hx, hy, er_ref=1.0, ur_ref=1.0,
+              er_trn=2.25, ur_trn=1.0):
+        """A sweep plan (rcwa_sweep_create) of a structure given as material
+        indices material[layer][iy][ix] (uint8, < num_materials)."""
+        return Sweep(self, material, thick, Lx, Ly, num_materials, hx, hy, er_ref, ur_ref,
+                     er_trn, ur_trn)
+
     def last_source_columns(self, count):
         buf = np.zeros(count, np.complex128)
         _check(lib().rcwa_last_source_columns(self.h, _ptr(buf), ctypes.c_size_t(count)))
@@ -407,6 +421,78 @@ def end_smatrix(which, hx, hy, Kx, Ky, Kz, er_ref, ur_ref, er_trn, ur_trn):
     _check(lib().rcwa_end_smatrix(which, PQ, ctypes.byref(par), _ptr(_cz(Kx)), _ptr(_cz(Ky)),
                                   _ptr(_cz(Kz)), *[_ptr(x) for x in S]))
     return tuple(x.reshape(n, n).T for x in S)
+
+
+class Sweep:
+    """A structure made of materials, for sweeps over wavelength and angle:
+    the convolution matrices of its (layer, material) masks are computed once
+    on the device, and each point mixes them with its material values
+    (rcwa_sweep_*)."""
+
+    def __init__(self, ctx, material, thick, Lx, Ly, num_materials, hx, hy, er_ref, ur_ref,
+                 er_trn, ur_trn):
+        self.ctx = ctx  # the plan uses the context: keep it alive
+        self.material = np.ascontiguousarray(material, dtype=np.uint8)
+        L, Ny, Nx = self.material.shape
+        self.thick = np.ascontiguousarray(thick, dtype=np.float64)
+        self.M, self.hx, self.hy = int(num_materials), int(hx), int(hy)
+        self.params = Params(hx, hy, _c(er_ref), _c(ur_ref), _c(er_trn), _c(ur_trn))
+        st = SweepStructure(Nx, Ny, L, self.M, Lx, Ly, self.material.ctypes.data,
+                            self.thick.ctypes.data)
+        self.h = ctypes.c_void_p()
+        _check(lib().rcwa_sweep_create(ctx.h, ctypes.byref(st), ctypes.byref(self.params),
+                                       ctypes.byref(self.h)))
+
+    def run(self, sources, eps, mu=None, precision=RCWA_FP64, fields=False):
+        """sources: per point a list of (lam0, theta, phi, pte, ptm), the same
+        count for every point; eps (and mu, None = 1): [points][materials].
+        Returns per point what Context.solve returns per problem."""
+        npt = len(sources)
+        nsrc = len(sources[0]) if npt else 1
+        if any(len(ss) != nsrc for ss in sources):
+            raise ValueError("every point needs the same number of sources")
+        P, Q = 2 * self.hx + 1, 2 * self.hy + 1
+        src = (Source * max(npt * nsrc, 1))(*[Source(*x) for ss in sources for x in ss])
+        e = _cz(eps, (npt, self.M))
+        m = None if mu is None else _cz(mu, (npt, self.M))
+        R = np.zeros((npt * nsrc, P * Q))
+        T = np.zeros((npt * nsrc, P * Q))
+        ra = np.zeros((npt * nsrc, 3 * P * Q), np.complex128) if fields else None
+        ta = np.zeros((npt * nsrc, 3 * P * Q), np.complex128) if fields else None
+        res = (Results * max(npt * nsrc, 1))(
+            *[Results(R[k].ctypes.data, T[k].ctypes.data, 0.0, 0.0,
+                      ra[k].ctypes.data if fields else None,
+                      ta[k].ctypes.data if fields else None) for k in range(npt * nsrc)])
+        _check(lib().rcwa_sweep_run(self.h, npt, src, nsrc, _ptr(e),
+                                    None if m is None else _ptr(m), precision, res))
+        out = []
+        for p in range(npt):
+            k = slice(p * nsrc, (p + 1) * nsrc)
+            tot = np.array([[res[p * nsrc + j].R_tot, res[p * nsrc + j].T_tot]
+                            for j in range(nsrc)])
+            one = (R[k].reshape(nsrc, P, Q).transpose(0, 2, 1),
+                   T[k].reshape(nsrc, P, Q).transpose(0, 2, 1), tot)
+            if fields:
+                one += (ra[k].reshape(nsrc, 3, P * Q), ta[k].reshape(nsrc, 3, P * Q))
+            out.append(one)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().rcwa_sweep_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

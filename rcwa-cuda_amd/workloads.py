@@ -135,6 +135,20 @@ def c2_sweep(nlam=200, grid=512, h=10, nslice=20, lam_lo=250.0, lam_hi=800.0, se
     return probs
 
 
+def c2_structure(nlam=200, grid=512, nslice=20, lam_lo=250.0, lam_hi=800.0, seed=2):
+    """The line grating of c2_sweep as material indices for a sweep plan:
+    material 0 is air, 1..5 the five dispersive materials.  Returns
+    (material uint8 [nslice][grid][grid], thick, lams, eps [nlam][6])."""
+    lams = np.linspace(lam_lo, lam_hi, nlam)
+    mats = c2_materials(lams, 5, seed)
+    masks, thick = c2_profile(grid, nslice=nslice)
+    material = np.zeros((nslice, grid, grid), np.uint8)
+    for k in range(nslice):
+        material[k][:, masks[k]] = 1 + (k * 5) // nslice
+    eps = np.concatenate([np.ones((nlam, 1), np.complex128), mats.T], axis=1)
+    return material, thick, lams, eps
+
+
 def random_shapes_layer(rng, grid, Lx, Ly, shapes=None, layer=0):
     """1-3 ellipses/rectangles with eps Re in [1,16], Im in [0,2].  If shapes
     is a list, the layer's (layer, kind, cx, cy, ax, ay, eps) entries are
