@@ -18,6 +18,10 @@ The JSON line reports:
                each), extrapolated to a full solve
   split        (--split) the split-precision form on the same wavelengths:
                solves/s and max |dR|, |dT| against this FP64 run
+  planner      (--planner) the same wavelengths through a sweep plan (the
+               grating as material masks whose convolution matrices are formed
+               once), in chunks of --lams: solves/s, plan set-up time and max
+               |dR|, |dT| against this FP64 run
 """
 import argparse
 import ctypes
@@ -182,6 +186,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lams", type=int, default=48, help="wavelengths per step")
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
+    ap.add_argument("--planner", action="store_true", help="also time the sweep-plan form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c5", action="store_true",
                     help="also time the 41x41-harmonic, 50-layer single structure")
@@ -313,6 +318,29 @@ def main():
         dT = max(np.abs(a[1] - b[1]).max() for ra, rb in zip(results, sres) for a, b in zip(ra, rb))
         out["split"] = {"solves_per_s": solves / selapsed, "speedup_vs_fp64": elapsed / selapsed,
                         "max_dR_vs_fp64": float(dR), "max_dT_vs_fp64": float(dT)}
+
+    if args.planner:
+        material, thick, lams, eps = W.c2_structure()
+        tidx = idx[args.warmup * args.lams:]
+        th = math.radians(65.0)
+        srcs = [[(lams[i], th, 0.0, 1.0, 0.0), (lams[i], th, 0.0, 0.0, 1.0)] for i in tidx]
+        t0 = time.perf_counter()
+        sw = ctx.sweep(material, thick, 120.0, 120.0, eps.shape[1], 10, 10)
+        t_plan = time.perf_counter() - t0
+        ctx.set_chunk(args.lams)
+        t0 = time.perf_counter()
+        pres = sw.run(srcs, eps[tidx])
+        pelapsed = time.perf_counter() - t0
+        ctx.set_chunk(0)
+        sw.close()
+        progress("sweep-plan form done")
+        flat = [r for step in results for r in step]
+        dR = max(np.abs(a[0] - b[0]).max() for a, b in zip(flat, pres))
+        dT = max(np.abs(a[1] - b[1]).max() for a, b in zip(flat, pres))
+        out["planner"] = {"solves_per_s": len(tidx) / pelapsed,
+                          "speedup_vs_fp64": elapsed / pelapsed,
+                          "plan_setup_ms": t_plan * 1e3,
+                          "max_dR_vs_fp64": float(dR), "max_dT_vs_fp64": float(dT)}
 
     # the stream and the large structure below use contexts of their own:
     # release this one's HBM arena and the sweep's grids first
