@@ -49,6 +49,7 @@ constexpr int QR_RUNNING = 0, QR_DONE = 1, QR_FAILED = 2;
 
 constexpr int APPLY_G = 8;     // sweeps per pass of qr_apply_kernel
 constexpr int APPLY_TC = 128;  // time steps per LDS-staged chunk of qr_apply_kernel
+constexpr int TREVC_TB = 32;   // rows per block of the blocked eigenvector solve
 constexpr int APPLY_NT = 256;
 
 __device__ __forceinline__ zc shfl_down_z(zc v, int off) {
@@ -1384,29 +1385,74 @@ __global__ void qr_state_init_kernel(int n, int batch, int* state, int* pending)
 
 __global__ void qr_clear_pending_kernel(int* pending) { *pending = 0; }
 
-__global__ __launch_bounds__(64) void trevc_kernel(int n, ZCMat Tm, ZMat Xm) {
-  extern __shared__ zc rhs[];
-  const int k = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
+// Eigenvectors of the upper triangular T, blocked.  X is upper triangular
+// with X[k][k] = 1 and, for i < k, (T[i][i] - T[k][k]) X[i][k] =
+// -sum_{i < m <= k} T[i][m] X[m][k] (back-substitution, the divisor raised to
+// smin where it is tiny, as LAPACK's ztrevc does).  Rows are taken in blocks
+// of TB from the bottom: trevc_diag_kernel solves block I for every column
+// right of its start (the columns are independent), then one ZGEMM subtracts
+// T[0:I0, I] X[I, I0:n] from the rows above, so the O(n^3) part of the work
+// runs on the matrix cores.  trevc_init_kernel sets X[k][k] = 1, X[i][k] =
+// -T[i][k] for the rows i < k of k's own block (the term m = k for the rows
+// of other blocks arrives with the update, through X[k][k] = 1) and 0
+// elsewhere.
+template <int TB>
+__global__ void trevc_init_kernel(int n, ZCMat Tm, ZMat Xm) {
+  const int b = blockIdx.y;
   const zc* T = Tm.p + (long long)b * Tm.stride;
-  zc* X = Xm.p + (long long)b * Xm.stride + (long long)k * Xm.ld;
-  const long long ld = Tm.ld;
-  const zc lk = T[k + k * ld];
-  const double smlnum = DBL_MIN * ((double)n / DBL_EPSILON);
-  const double smin = fmax(DBL_EPSILON * zabs1(lk), smlnum);
-  for (int i = lane; i < n; i += 64) {
-    if (i < k) rhs[i] = zneg(T[i + k * ld]);
-    else X[i] = zmake(i == k ? 1.0 : 0.0, 0.0);
+  zc* X = Xm.p + (long long)b * Xm.stride;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < (long long)n * n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % n), k = (int)(idx / n);
+    zc v = zmake(0, 0);
+    if (i == k)
+      v = zmake(1, 0);
+    else if (i < k && i >= (k / TB) * TB)
+      v = zneg(T[i + (long long)k * Tm.ld]);
+    X[i + (long long)k * Xm.ld] = v;
+  }
+}
+
+// Rows I0 .. I0+TB-1 of every column k > I0: one thread per column, its TB
+// unknowns in registers, the diagonal block of T in LDS (read by all lanes
+// at one address at a time).
+template <int TB>
+__global__ __launch_bounds__(256) void trevc_diag_kernel(int n, int I0, ZCMat Tm, ZMat Xm) {
+  __shared__ zc tb[TB * TB];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const zc* T = Tm.p + (long long)b * Tm.stride;
+  zc* X = Xm.p + (long long)b * Xm.stride;
+  const long long ld = Tm.ld, ldx = Xm.ld;
+  const int nbk = min(TB, n - I0);
+  for (int e = tid; e < TB * TB; e += 256) {
+    const int i = e % TB, j = e / TB;
+    tb[e] = (i < nbk && j < nbk) ? T[(I0 + i) + (long long)(I0 + j) * ld] : zmake(0, 0);
   }
   __syncthreads();
-  for (int i = k - 1; i >= 0; --i) {
-    zc d = zsub(T[i + i * ld], lk);
-    if (zabs1(d) < smin) d = zmake(smin, 0.0);
-    const zc xi = zdiv(rhs[i], d);
-    if (lane == 0) X[i] = xi;
-    const zc* tc = T + i * ld;
-    for (int m = lane; m < i; m += 64) rhs[m] = zsub(rhs[m], zmul(tc[m], xi));
-    __syncthreads();
+  const int k = I0 + 1 + blockIdx.x * 256 + tid;
+  if (k >= n) return;
+  const zc lk = T[k + (long long)k * ld];
+  const double smlnum = DBL_MIN * ((double)n / DBL_EPSILON);
+  const double smin = fmax(DBL_EPSILON * zabs1(lk), smlnum);
+  const int iend = min(k, I0 + nbk) - I0;  // unknown rows I0 .. I0+iend-1
+  zc* xc = X + (long long)k * ldx + I0;
+  zc x[TB];
+#pragma unroll
+  for (int i = 0; i < TB; ++i) x[i] = i < iend ? xc[i] : zmake(0, 0);
+#pragma unroll
+  for (int i = TB - 1; i >= 0; --i) {
+    if (i < iend) {
+      zc d = zsub(tb[i + i * TB], lk);
+      if (zabs1(d) < smin) d = zmake(smin, 0.0);
+      const zc xi = zdiv(x[i], d);
+      x[i] = xi;
+#pragma unroll
+      for (int m = 0; m < i; ++m) x[m] = zsub(x[m], zmul(tb[m + i * TB], xi));
+    }
   }
+#pragma unroll
+  for (int i = 0; i < TB; ++i)
+    if (i < iend) xc[i] = x[i];
 }
 
 __global__ void diag_kernel(int n, ZCMat Tm, zc* w) {
@@ -1524,8 +1570,31 @@ int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, in
 
 int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch) {
   if (n <= 0 || batch <= 0) return 0;
-  hipLaunchKernelGGL(trevc_kernel, dim3(n, batch), dim3(64), (size_t)n * sizeof(zc), s, n, T, X);
-  return (int)hipGetLastError();
+  constexpr int TB = TREVC_TB;
+  long long blocks = ((long long)n * n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL((trevc_init_kernel<TB>), dim3((unsigned)blocks, batch), dim3(256), 0, s, n,
+                     T, X);
+  if (int e = (int)hipGetLastError()) return e;
+  const zc mone = zmake(-1, 0), one = zmake(1, 0);
+  for (int I0 = ((n - 1) / TB) * TB; I0 >= 0; I0 -= TB) {
+    if (n - I0 - 1 > 0) {
+      hipLaunchKernelGGL((trevc_diag_kernel<TB>), dim3((n - I0 - 1 + 255) / 256, batch),
+                         dim3(256), 0, s, n, I0, T, X);
+      if (int e = (int)hipGetLastError()) return e;
+    }
+    const int nbk = std::min(TB, n - I0);
+    if (I0 > 0) {
+      // X[0:I0, I0:n] -= T[0:I0, I0:I0+nbk] X[I0:I0+nbk, I0:n]
+      const ZCMat Ab{T.p + (long long)I0 * T.ld, T.ld, T.stride};
+      const ZCMat Bb{X.p + I0 + (long long)I0 * X.ld, X.ld, X.stride};
+      const ZMat Cb{X.p + (long long)I0 * X.ld, X.ld, X.stride};
+      if (int e = zgemm_batched(s, I0, n - I0, nbk, mone, Ab, Bb, one,
+                                ZCMat{Cb.p, Cb.ld, Cb.stride}, Cb, batch))
+        return e;
+    }
+  }
+  return 0;
 }
 
 size_t eig_workspace_bytes(int n, int batch) {
