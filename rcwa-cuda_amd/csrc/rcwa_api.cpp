@@ -67,6 +67,7 @@ rcwa_status Ctx::init(int dev, size_t bud) {
   device = dev;
   CKH(hipSetDevice(dev));
   CKH(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  CKH(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
   size_t fr = 0, tot = 0;
   CKH(hipMemGetInfo(&fr, &tot));
   budget = bud ? bud : (size_t)(0.8 * (double)fr);
@@ -83,6 +84,8 @@ void Ctx::release() {
   arena_cap = 0;
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (stream2) (void)hipStreamDestroy(stream2);
+  stream2 = nullptr;
   if (stream) (void)hipStreamDestroy(stream);
   stream = nullptr;
 }
@@ -96,6 +99,7 @@ rcwa_status Ctx::reserve(size_t bytes) {
   if (arena_cap < bytes) {
     CKH(hipSetDevice(device));
     CKH(hipStreamSynchronize(stream));
+    if (stream2) CKH(hipStreamSynchronize(stream2));
     if (arena) CKH(hipFree(arena));
     arena = nullptr;
     arena_cap = 0;
@@ -581,34 +585,36 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     CK(convmat_gather(s, F, Nx, Ny, hx, hy, ZMat{urc, N, N2}, L, map));
   }
   CKH(hipEventRecord(c.ev[1], s));
-  // ---- SMatrixLayer for every layer (main.cpp:151)
-  // RCWA_PROGRESS set: a line on stderr as each layer batch completes (for
-  // long single solves; the wait it adds is one stream synchronisation per
-  // batch)
-  const bool progress = getenv("RCWA_PROGRESS") != nullptr;
-  for (int l0 = 0; l0 < TL; l0 += Bl) {
-    const int b = std::min(Bl, TL - l0);
-    // sweep points: inverses formed per row above; [[mu]] = I is its own
-    const ZCMat none_m{nullptr, 0, 0};
-    const ZCMat erci_pre = erci_all ? ZCMat{erci_all + (size_t)l0 * N2, N, N2} : none_m;
-    const ZCMat urci_pre = (plan && mu_unit) ? ZCMat{urc + (size_t)l0 * N2, N, N2}
-                           : urci_all        ? ZCMat{urci_all + (size_t)l0 * N2, N, N2}
-                                             : none_m;
-    st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
-                     dKx + (size_t)l0 * N, dKy + (size_t)l0 * N, dv0 + (size_t)l0 * 4 * N,
-                     dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
-                     ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
-                     ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
-                     info_eig + l0, info_sing + l0, erci_pre, urci_pre);
-    if (st) return st;
-    if (progress) {
-      CKH(hipStreamSynchronize(s));
-      fprintf(stderr, "[rcwa] layer matrices %d/%d done\n", l0 + b, TL);
-      fflush(stderr);
+  // ---- fold (main.cpp:143-170), on its own stream in the FP64 form: fold
+  // step l needs only layer l's S-matrices, so it runs as soon as the layer
+  // batch holding them is done, beside the next batch's layer stage (the
+  // layer-major order puts layer l of every job in one batch or two).  The
+  // split form keeps one stream: its products share one slice workspace.
+  const bool overlap = c.precision == RCWA_FP64 && c.stream2 != nullptr;
+  hipStream_t fs = overlap ? c.stream2 : s;
+  std::vector<hipEvent_t> evs;  // evs[0]: inputs uploaded; then one per layer batch
+  struct EventsGuard {
+    std::vector<hipEvent_t>& v;
+    ~EventsGuard() {
+      for (hipEvent_t e : v) (void)hipEventDestroy(e);
     }
-  }
-  CKH(hipEventRecord(c.ev[4], s));
-  // ---- fold (main.cpp:143-170) for the two source columns
+  } evs_guard{evs};
+  auto fence = [&](hipStream_t from, hipStream_t to) -> rcwa_status {
+    if (from == to) return RCWA_OK;
+    hipEvent_t e;
+    CKH(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    evs.push_back(e);
+    CKH(hipEventRecord(e, from));
+    CKH(hipStreamWaitEvent(to, e, 0));
+    return RCWA_OK;
+  };
+  auto fgemm = [&](int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D_,
+                   ZMat C_, int bt) {
+    return overlap ? zgemm_batched(fs, M_, N_, K_, al, A_, B_, be, D_, C_, bt)
+                   : c.gemm(M_, N_, K_, al, A_, B_, be, D_, C_, bt);
+  };
+  if ((st = fence(s, fs)) != RCWA_OK) return st;
+  // The fold for the two source columns:
   // S_glob = S_ref * S_0 * ... * S_{L-1} * S_trn (Redheffer products,
   // rcwa.cpp:588-615), associated left to right.  The efficiencies need only
   // the columns of S_glob's S11 and S21 at the two source entries
@@ -643,49 +649,81 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
   auto Mv = [&](zc* p) { return ZMat{p, n, 2LL * n}; };
   auto CMv = [&](const zc* p) { return ZCMat{p, n, 2LL * n}; };
-  CK(zset_identity(s, n, ZMat{Id, n, n2}, 1, one));
+  CK(zset_identity(fs, n, ZMat{Id, n, n2}, 1, one));
   // P = S_ref: full S12, S22; columns of S11, S21
-  CK(expand_block_diag(s, N, dends + 0 * N, 32LL * N, Mt(X11), G));
-  CK(expand_block_diag(s, N, dends + 4 * (size_t)N, 32LL * N, Mt(P12), G));
-  CK(expand_block_diag(s, N, dends + 8 * (size_t)N, 32LL * N, Mt(X21), G));
-  CK(expand_block_diag(s, N, dends + 12 * (size_t)N, 32LL * N, Mt(P22), G));
-  CK(gather_cols(s, n, CMt(X11), dcolidx, 2, va, G));
-  CK(gather_cols(s, n, CMt(X21), dcolidx, 2, vb, G));
+  CK(expand_block_diag(fs, N, dends + 0 * N, 32LL * N, Mt(X11), G));
+  CK(expand_block_diag(fs, N, dends + 4 * (size_t)N, 32LL * N, Mt(P12), G));
+  CK(expand_block_diag(fs, N, dends + 8 * (size_t)N, 32LL * N, Mt(X21), G));
+  CK(expand_block_diag(fs, N, dends + 12 * (size_t)N, 32LL * N, Mt(P22), G));
+  CK(gather_cols(fs, n, CMt(X11), dcolidx, 2, va, G));
+  CK(gather_cols(fs, n, CMt(X21), dcolidx, 2, vb, G));
   // P <- P * X for the first na jobs
   auto step = [&](int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22,
                   bool columns_only) -> rcwa_status {
-    CK(c.gemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
-    CK(gj_inverse_batched(s, n, Mt(Fm), na, gjf, info_sing));
+    CK(fgemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
+    CK(gj_inverse_batched(fs, n, Mt(Fm), na, gjf, info_sing));
     // the columns, with the old P12 and P22
-    CK(c.gemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
-    CK(c.gemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
-    CK(c.gemm(n, 2, n, one, CMt(P12), CMv(vw), one, CMv(va), Mv(va), na));
-    CK(c.gemm(n, 2, n, one, CMt(P22), CMv(vw), one, CMv(vb), Mv(vy), na));
-    CK(c.gemm(n, 2, n, one, x21, CMv(vy), zero, none, Mv(vb), na));
+    CK(fgemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
+    CK(fgemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
+    CK(fgemm(n, 2, n, one, CMt(P12), CMv(vw), one, CMv(va), Mv(va), na));
+    CK(fgemm(n, 2, n, one, CMt(P22), CMv(vw), one, CMv(vb), Mv(vy), na));
+    CK(fgemm(n, 2, n, one, x21, CMv(vy), zero, none, Mv(vb), na));
     if (columns_only) return RCWA_OK;
-    CK(c.gemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
-    CK(c.gemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
-    CK(c.gemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
-    CK(c.gemm(n, n, n, one, x21, CMt(Hm), one, x22, Mt(P22n), na));
+    CK(fgemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
+    CK(fgemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
+    CK(fgemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
+    CK(fgemm(n, n, n, one, x21, CMt(Hm), one, x22, Mt(P22n), na));
     if (na < G) {  // the jobs already past their last layer keep their P
-      CK(zcopy_batched(s, n, n, CMt(P12 + (size_t)na * n2), Mt(P12n + (size_t)na * n2), G - na));
-      CK(zcopy_batched(s, n, n, CMt(P22 + (size_t)na * n2), Mt(P22n + (size_t)na * n2), G - na));
+      CK(zcopy_batched(fs, n, n, CMt(P12 + (size_t)na * n2), Mt(P12n + (size_t)na * n2), G - na));
+      CK(zcopy_batched(fs, n, n, CMt(P22 + (size_t)na * n2), Mt(P22n + (size_t)na * n2), G - na));
     }
     std::swap(P12, P12n);
     std::swap(P22, P22n);
     return RCWA_OK;
   };
-  for (int l = 0; l < Lmax; ++l) {
-    const ZCMat x11 = CMt(S11l + (size_t)off[l] * n2), x12 = CMt(S12l + (size_t)off[l] * n2);
-    st = step(act[l], x11, x12, x12, x11, false);  // layers: S21 = S12, S22 = S11
+  int lf = 0;  // next fold step
+  // ---- SMatrixLayer for every layer (main.cpp:151)
+  // RCWA_PROGRESS set: a line on stderr as each layer batch completes (for
+  // long single solves; the wait it adds is one stream synchronisation per
+  // batch)
+  const bool progress = getenv("RCWA_PROGRESS") != nullptr;
+  for (int l0 = 0; l0 < TL; l0 += Bl) {
+    const int b = std::min(Bl, TL - l0);
+    // sweep points: inverses formed per row above; [[mu]] = I is its own
+    const ZCMat none_m{nullptr, 0, 0};
+    const ZCMat erci_pre = erci_all ? ZCMat{erci_all + (size_t)l0 * N2, N, N2} : none_m;
+    const ZCMat urci_pre = (plan && mu_unit) ? ZCMat{urc + (size_t)l0 * N2, N, N2}
+                           : urci_all        ? ZCMat{urci_all + (size_t)l0 * N2, N, N2}
+                                             : none_m;
+    st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
+                     dKx + (size_t)l0 * N, dKy + (size_t)l0 * N, dv0 + (size_t)l0 * 4 * N,
+                     dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
+                     ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
+                     ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
+                     info_eig + l0, info_sing + l0, erci_pre, urci_pre);
     if (st) return st;
+    if (progress) {
+      CKH(hipStreamSynchronize(s));
+      fprintf(stderr, "[rcwa] layer matrices %d/%d done\n", l0 + b, TL);
+      fflush(stderr);
+    }
+    if (lf < Lmax && off[lf + 1] <= l0 + b) {
+      if ((st = fence(s, fs)) != RCWA_OK) return st;
+      for (; lf < Lmax && off[lf + 1] <= l0 + b; ++lf) {
+        const ZCMat x11 = CMt(S11l + (size_t)off[lf] * n2), x12 = CMt(S12l + (size_t)off[lf] * n2);
+        st = step(act[lf], x11, x12, x12, x11, false);  // layers: S21 = S12, S22 = S11
+        if (st) return st;
+      }
+    }
   }
+  CKH(hipEventRecord(c.ev[4], s));
   // * S_trn: only the columns
-  CK(expand_block_diag(s, N, dends + 16 * (size_t)N, 32LL * N, Mt(X11), G));
-  CK(expand_block_diag(s, N, dends + 24 * (size_t)N, 32LL * N, Mt(X21), G));
+  CK(expand_block_diag(fs, N, dends + 16 * (size_t)N, 32LL * N, Mt(X11), G));
+  CK(expand_block_diag(fs, N, dends + 24 * (size_t)N, 32LL * N, Mt(X21), G));
   st = step(G, CMt(X11), CMt(X11), CMt(X21), CMt(X11), true);
   if (st) return st;
-  CKH(hipEventRecord(c.ev[5], s));
+  if ((st = fence(fs, s)) != RCWA_OK) return st;
+  CKH(hipEventRecord(c.ev[5], s));  // ms_fold: the fold's time not hidden by the layers
   // ---- S11 / S21 columns at the source entries (rcwa.cpp:512-514, 535, 553)
   std::vector<cplx> hcols((size_t)G * 4 * n);
   CKH(hipMemcpyAsync(hcols.data(), dcols, hcols.size() * sizeof(zc), hipMemcpyDeviceToHost, s));

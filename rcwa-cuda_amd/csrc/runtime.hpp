@@ -19,6 +19,7 @@ int zgemm_split_batched_ws(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // the fold of the batched solve, beside the layer stage
   size_t budget = 0;
   char* arena = nullptr;
   void* split_ws = nullptr;  // operand slices of the split-precision GEMM
