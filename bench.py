@@ -184,7 +184,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--lams", type=int, default=48, help="wavelengths per step")
+    # 51 wavelengths x 20 layers = 1020 layer eigenproblems, solved in two
+    # launches of 510: about two full rounds of one matrix per CU on 256 CUs
+    # (48 gives 480 per launch, with 32 CUs idle in the second round)
+    ap.add_argument("--lams", type=int, default=51, help="wavelengths per step")
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--planner", action="store_true", help="also time the sweep-plan form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
