@@ -6,18 +6,24 @@
 // H_j = I - tau_j v_j v_j^H (v_j zero above row j+1, one at row j+1) are
 // combined as H_p ... H_{p+nb-1} = I - V T V^H (compact WY, T upper
 // triangular).  The panel kernel (one workgroup per matrix) produces V, T and
-// Y = A0 V T, where A0 is the matrix at the start of the panel:
-//   column j is brought up to date with the earlier reflectors of the panel,
-//     a_j <- (I - V T^H V^H)(A0 e_j - Y V^H e_j);
+// the rows p+1.. of Y = A0 V T, where A0 is the matrix at the start of the
+// panel:
+//   rows p+1.. of column j are brought up to date with the earlier
+//   reflectors of the panel, a_j <- (I - V T^H V^H)(A0 e_j - Y V^H e_j);
 //   its reflector is generated from rows j+1..n-1;
 //   y_j = tau_j (A0[:, j+1:] v_j - Y (V^H v_j)), t_j = -tau_j T (V^H v_j).
-// The columns right of the panel then receive the whole block at once on the
-// matrix cores:
+// Rows 0..p see only the right transforms, so they are left to the matrix
+// cores after the panel (as LAPACK's zlahr2/zgehrd do):
+//   Y[:p+1, :]      = (A0[:p+1, p+1:] V[p+1:, :]) T
+//   A[:p+1, p:p+nb] -= Y[:p+1, :] V[p:p+nb, :]^H
+// The columns right of the panel then receive the whole block at once:
 //   A[:, p+nb:]    -= Y V[p+nb:, :]^H                  (A <- A H)
 //   A[p+1:, p+nb:] -= V (T^H (V^H A[p+1:, p+nb:]))      (A <- H^H A)
 //   Q[:, p+1:]     -= (Q[:, p+1:] V T) V^H              (Q <- Q H)
-// The reduction's only memory-bound part is the product A0[:, j+1:] v_j, one
-// read of the trailing matrix per column; everything else is ZGEMM.
+// The reduction's only memory-bound part is the product A0[p+1:, j+1:] v_j,
+// one read of the trailing block per column (n^3/3 entries over the
+// reduction, against n^3/2 when every row is formed in the panel); everything
+// else is ZGEMM.
 #include <algorithm>
 #include "common.hpp"
 #include "kernels.hpp"
@@ -54,7 +60,8 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
   for (int jj = 0; jj < nbe; ++jj) {
     const int j = p + jj;
     // (a) right transforms of the panel so far: b = A0 e_j - Y V(j, :)^H
-    for (int i = tid; i < n; i += PT) {
+    // (rows p+1.. only: rows 0..p are updated after the panel)
+    for (int i = p + 1 + tid; i < n; i += PT) {
       zc v = A[i + (long long)j * lda];
       for (int q = 0; q < jj; ++q) v = zsub(v, zmul(Y[i + (size_t)q * n], zconj(V[j + (size_t)q * n])));
       bcol[i] = v;
@@ -107,18 +114,19 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
     }
     __syncthreads();
     const zc tau = s_tau, scale = s_scale;
-    // store the finished column j and the reflector v (in bcol and V(:, jj))
+    // store rows p+1.. of the finished column j and the reflector v (in bcol
+    // and V(:, jj), zero above row j+1)
     for (int i = tid; i < n; i += PT) {
       zc a, v;
       if (i <= j) { a = bcol[i]; v = zmake(0, 0); }
       else if (i == j + 1) { a = s_beta; v = zmake(1, 0); }
       else { a = zmake(0, 0); v = zmul(bcol[i], scale); }
-      A[i + (long long)j * lda] = a;
+      if (i > p) A[i + (long long)j * lda] = a;
       V[i + (size_t)jj * n] = v;
       bcol[i] = v;
     }
     __syncthreads();
-    // (c) y = A0[:, j+1:] v ; z = V(:, :jj)^H v
+    // (c) y = A0[p+1:, j+1:] v ; z = V(:, :jj)^H v
     for (int q = wave; q < jj; q += nw) {
       zc acc = zmake(0, 0);
       for (int i = j + 1 + lane; i < n; i += 64)
@@ -135,7 +143,7 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
       const zc* col = A + (long long)c * lda;
 #pragma unroll
       for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
-        const int i = tid + r * PT;
+        const int i = p + 1 + tid + r * PT;
         if (i < n) {
           zc a[8];
 #pragma unroll
@@ -150,14 +158,14 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
       const zc* col = A + (long long)c * lda;
 #pragma unroll
       for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
-        const int i = tid + r * PT;
+        const int i = p + 1 + tid + r * PT;
         if (i < n) yacc[r] = zfma(col[i], vc, yacc[r]);
       }
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
-      const int i = tid + r * PT;
+      const int i = p + 1 + tid + r * PT;
       if (i < n) {
         zc y = yacc[r];
         for (int q = 0; q < jj; ++q) y = zsub(y, zmul(Y[i + (size_t)q * n], zq[q]));
@@ -248,6 +256,23 @@ int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, 
     const int m = n - c0;                     // trailing columns
     const int r1 = p + 1, mr = n - r1;        // rows of V that are nonzero
     int e;
+    // rows 0..p: Y[:r1, :] = (A0[:r1, r1:] V[r1:, :]) T, then
+    // A[:r1, p:c0] -= Y[:r1, :] V[p:c0, :]^H (row p of V is zero)
+    if ((e = zgemm_batched(s, r1, nbe, mr, one, ZCMat{A.p + (long long)r1 * A.ld, A.ld, A.stride},
+                           ZCMat{V + r1, n, (long long)nNB}, zero, none,
+                           ZMat{QV, r1, (long long)nNB}, batch)))
+      return e;
+    if ((e = zgemm_batched(s, r1, nbe, nbe, one, ZCMat{QV, r1, (long long)nNB},
+                           ZCMat{T, NB, (long long)NB * NB}, zero, none,
+                           ZMat{Y, n, (long long)nNB}, batch)))
+      return e;
+    hipLaunchKernelGGL(conj_transpose_kernel, g2((long long)nbe * nbe, batch), dim3(256), 0, s,
+                       nbe, nbe, V + p, (long long)n, (long long)nNB, T2, (long long)NB * NB);
+    if ((e = zgemm_batched(s, r1, nbe, nbe, mone, ZCMat{Y, n, (long long)nNB},
+                           ZCMat{T2, nbe, (long long)NB * NB}, one,
+                           ZCMat{A.p + (long long)p * A.ld, A.ld, A.stride},
+                           ZMat{A.p + (long long)p * A.ld, A.ld, A.stride}, batch)))
+      return e;
     if (m > 0) {
       // A[:, c0:] -= Y V[c0:, :]^H
       hipLaunchKernelGGL(conj_transpose_kernel, g2((long long)m * nbe, batch), dim3(256), 0, s, m,
