@@ -233,21 +233,52 @@ __global__ __launch_bounds__(REG_THREADS) void gj_panel_reg_kernel(int n, int k0
 }
 
 // Applies the row swaps of panel k0..k0+nb-1 (in order) to the columns
-// outside the panel; one thread per column.
-__global__ void gj_swap_kernel(int n, int k0, int nb, ZMat Mm, const int* perm) {
+// outside the panel; one thread per column.  The nb swaps touch at most 2 nb
+// rows; their net permutation (row dst[k] receives the old row src[k]) is
+// formed once per workgroup, so that every load of a column is independent of
+// its stores and the loads issue together.
+constexpr int GJ_SWAP_MAX = 32;  // rows touched by the swaps of one panel (nb <= 16)
+__global__ __launch_bounds__(256) void gj_swap_kernel(int n, int k0, int nb, ZMat Mm,
+                                                      const int* perm) {
+  __shared__ int s_dst[GJ_SWAP_MAX], s_src[GJ_SWAP_MAX];
+  __shared__ int s_m;
   const int b = blockIdx.y;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n || (j >= k0 && j < k0 + nb)) return;
-  zc* col = Mm.p + (long long)b * Mm.stride + (long long)j * Mm.ld;
   const int* pv = perm + (long long)b * n;
-  for (int q = 0; q < nb; ++q) {
-    const int c = k0 + q, pr = pv[c];
-    if (pr != c) {
-      const zc x = col[c];
-      col[c] = col[pr];
-      col[pr] = x;
+  if (threadIdx.x == 0) {
+    int m = 0;
+    for (int q = 0; q < nb; ++q) {
+      const int c = k0 + q, pr = pv[c];
+      if (pr == c) continue;
+      int ic = -1, ip = -1;
+      for (int k = 0; k < m; ++k) {
+        if (s_dst[k] == c) ic = k;
+        if (s_dst[k] == pr) ip = k;
+      }
+      if (ic < 0) { ic = m; s_dst[m] = c; s_src[m] = c; ++m; }
+      if (ip < 0) { ip = m; s_dst[m] = pr; s_src[m] = pr; ++m; }
+      const int t = s_src[ic];
+      s_src[ic] = s_src[ip];
+      s_src[ip] = t;
     }
+    s_m = m;
   }
+  __syncthreads();
+  const int m = s_m;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == 0 || j >= n || (j >= k0 && j < k0 + nb)) return;
+  zc* col = Mm.p + (long long)b * Mm.stride + (long long)j * Mm.ld;
+  // entries past m repeat the last pair (no branch, so the array stays in
+  // registers and the loads issue together): they store the value the last
+  // pair stores, to the same row
+  double xr[GJ_SWAP_MAX], xi[GJ_SWAP_MAX];
+#pragma unroll
+  for (int k = 0; k < GJ_SWAP_MAX; ++k) {
+    const zc v = col[s_src[min(k, m - 1)]];
+    xr[k] = v.x;
+    xi[k] = v.y;
+  }
+#pragma unroll
+  for (int k = 0; k < GJ_SWAP_MAX; ++k) col[s_dst[min(k, m - 1)]] = zmake(xr[k], xi[k]);
 }
 
 // Gp(i, r) = A(i, k0+r) - delta(i, k0+r) (n x nb);
@@ -261,20 +292,20 @@ __global__ void gj_prep_kernel(int n, int k0, int nb, int j0, int ncols, ZCMat M
   zc* G = Gp + (long long)b * n * nb;
   zc* T = Tm + (long long)b * ncols * nb;
   const int kend = min(n, k0 + nb), w = kend - k0;
-  const long long tot = (long long)max(n, ncols) * w;
-  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < tot;
-       idx += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % max(n, ncols)), r = (int)(idx / max(n, ncols));
-    if (i < n) {
-      zc g = A[i + (long long)(k0 + r) * ld];
-      if (i == k0 + r) g.x -= 1.0;
-      G[i + (long long)r * n] = g;
-    }
-    if (i < ncols) {
-      const int j = j0 + i;
-      T[r + (long long)i * w] =
-          (j >= k0 && j < kend) ? zmake(0, 0) : A[(k0 + r) + (long long)j * ld];
-    }
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long first = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // Gp: rows fastest (consecutive lanes read consecutive rows of a column)
+  for (long long idx = first; idx < (long long)n * w; idx += stride) {
+    const int i = (int)(idx % n), r = (int)(idx / n);
+    zc g = A[i + (long long)(k0 + r) * ld];
+    if (i == k0 + r) g.x -= 1.0;
+    G[i + (long long)r * n] = g;
+  }
+  // Tm: the panel's rows, r fastest (the w rows of one column are contiguous)
+  for (long long idx = first; idx < (long long)ncols * w; idx += stride) {
+    const int r = (int)(idx % w), i = (int)(idx / w);
+    const int j = j0 + i;
+    T[r + (long long)i * w] = (j >= k0 && j < kend) ? zmake(0, 0) : A[(k0 + r) + (long long)j * ld];
   }
 }
 
@@ -418,6 +449,7 @@ int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* i
         else
           hipLaunchKernelGGL((gj_panel_reg_kernel<4, 4>), dim3(batch), dim3(REG_THREADS), 0, s,
                              n, k0, w, M, perm, info);
+        if (2 * w > GJ_SWAP_MAX) return (int)hipErrorInvalidValue;
         hipLaunchKernelGGL(gj_swap_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n, k0,
                            w, M, perm);
         if (W > w) {

@@ -67,6 +67,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
       acc3[i][j] = v4d{0, 0, 0, 0};
     }
 
+  // The addend D is loaded before the K loop, so that its loads are in flight
+  // under the products: for the rank-16 to rank-64 updates (Gauss-Jordan,
+  // Hessenberg, eigenvector blocks) reading and writing C is most of the work.
+  const zc* Dp = D.p ? D.p + (long long)b * D.stride : nullptr;
+  const bool use_d = (beta.x != 0.0 || beta.y != 0.0) && Dp != nullptr;
+  zc dv[2][2][4];
+  if (use_d) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + wn * 32 + j * 16 + li;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * 32 + i * 16 + 4 * r + lk;
+          dv[i][j][r] = (row < M && col < N) ? Dp[row + (long long)col * D.ld] : zmake(0, 0);
+        }
+      }
+  }
+
   zc ra[4], rb[4];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -129,8 +149,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
   }
 
   zc* Cp = C.p + (long long)b * C.stride;
-  const zc* Dp = D.p ? D.p + (long long)b * D.stride : nullptr;
-  const bool use_d = (beta.x != 0.0 || beta.y != 0.0) && Dp != nullptr;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -143,7 +161,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
         if (row >= M) continue;
         const double p1 = acc1[i][j][r], p2 = acc2[i][j][r];
         zc v = zmul(alpha, zmake(p1 - p2, acc3[i][j][r] - (p1 + p2)));
-        if (use_d) v = zfma(beta, Dp[row + (long long)col * D.ld], v);
+        if (use_d) v = zfma(beta, dv[i][j][r], v);
         Cp[row + (long long)col * C.ld] = v;
       }
     }

@@ -51,6 +51,21 @@ def test_inverse_large(ctx, n):
         assert np.abs(x - np.linalg.inv(a)).max() / np.abs(x).max() < 1e-11
 
 
+@pytest.mark.parametrize("n", [40, 300, 882])
+def test_inverse_random_pivots(ctx, n):
+    """Unshifted Gaussian matrices: nearly every column pivots, so each panel's
+    row swaps (up to 2 x 16 rows, applied to the other columns as one
+    permutation) are exercised; errors scale with the condition number."""
+    rng = np.random.default_rng(7 + n)
+    A = [rnd(rng, n, n) for _ in range(3)]
+    X = ctx.inverse(A)
+    for a, x in zip(A, X):
+        ref = np.linalg.inv(a)
+        cond = np.linalg.cond(a)
+        assert np.abs(a @ x - np.eye(n)).max() < 1e-14 * n * cond
+        assert np.abs(x - ref).max() / np.abs(ref).max() < 1e-14 * n * cond
+
+
 def test_inverse_needs_pivoting(ctx):
     # zero leading entry forces a row swap at the first step
     A = np.array([[0, 1, 2], [3, 4, 5], [6, 7, 9]], np.complex128) + 0.5j * np.eye(3)
