@@ -1637,17 +1637,19 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
   if ((e = zgemm_batched(s, n, n, n, one, ZCMat{Ap, n, nn}, ZCMat{Zp, n, nn}, zero,
                          ZCMat{nullptr, 0, 0}, X, batch)))
     return e;
+  // (only the tiles meeting T's upper triangle: nothing reads T below it)
   if ((e = zconj_transpose_batched(s, n, n, ZCMat{Zp, n, nn}, ZMat{Ap, n, nn}, batch))) return e;
-  if ((e = zgemm_batched(s, n, n, n, one, ZCMat{Ap, n, nn}, ZCMat{Xp, n, nn}, zero,
-                         ZCMat{nullptr, 0, 0}, A, batch)))
+  if ((e = zgemm_batched_tri(s, n, n, n, one, ZCMat{Ap, n, nn}, ZCMat{Xp, n, nn}, zero,
+                             ZCMat{nullptr, 0, 0}, A, batch, ZGEMM_TRI_C)))
     return e;
   if (marks) (void)hipEventRecord(marks[2], s);
   if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch))) return e;
   hipLaunchKernelGGL(diag_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n,
                      ZCMat{A.p, A.ld, A.stride}, w);
-  if ((e = zgemm_batched(s, n, n, n, zmake(1, 0), ZCMat{Zp, n, (long long)n * n},
-                         ZCMat{Xp, n, (long long)n * n}, zmake(0, 0), ZCMat{nullptr, 0, 0}, W,
-                         batch)))
+  // W = Z X, X upper triangular (the eigenvectors of T)
+  if ((e = zgemm_batched_tri(s, n, n, n, zmake(1, 0), ZCMat{Zp, n, (long long)n * n},
+                             ZCMat{Xp, n, (long long)n * n}, zmake(0, 0), ZCMat{nullptr, 0, 0}, W,
+                             batch, ZGEMM_TRI_B)))
     return e;
   if ((e = znormalize_cols(s, n, W, batch))) return e;
   if (marks) (void)hipEventRecord(marks[3], s);

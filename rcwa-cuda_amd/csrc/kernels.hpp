@@ -7,6 +7,13 @@
 // zgemm.hip ------------------------------------------------------------------
 int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
                   ZCMat D, ZMat C, int batch);
+// Structure flags of zgemm_batched_tri: ZGEMM_TRI_C forms only the 64 x 64
+// tiles of C that meet its upper triangle (the tiles strictly below are not
+// written); ZGEMM_TRI_B takes B as upper triangular (its entries below the
+// diagonal are not read).
+constexpr int ZGEMM_TRI_C = 1, ZGEMM_TRI_B = 2;
+int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                      ZCMat D, ZMat C, int batch, int tri);
 
 // linalg.hip -----------------------------------------------------------------
 // In-place inverse of a batch of n x n matrices by blocked Gauss-Jordan

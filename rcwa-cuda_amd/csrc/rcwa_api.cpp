@@ -71,6 +71,7 @@ rcwa_status Ctx::init(int dev, size_t bud) {
   size_t fr = 0, tot = 0;
   CKH(hipMemGetInfo(&fr, &tot));
   budget = bud ? bud : (size_t)(0.8 * (double)fr);
+  CKH(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   for (auto& e : ev) CKH(hipEventCreate(&e));
   return RCWA_OK;
 }
@@ -410,6 +411,15 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   int Bl = std::max(TL, 1);
   while (Bl > 1 && chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra > c.budget)
     Bl = (Bl + 1) / 2;
+  // The eigensolver's kernels hold at most two matrices per CU at once, so a
+  // layer batch beyond 2 x CUs only queues a second round behind the first;
+  // for matrices whose work outweighs a launch (n >= 256), the layers are
+  // split into batches of equal size of at most that many.  More batches also
+  // let the fold of the earlier ones run beside the later ones.
+  if (2 * N >= 256 && Bl > 2 * c.cus) {
+    const int nb = (TL + 2 * c.cus - 1) / (2 * c.cus);
+    Bl = (TL + nb - 1) / nb;
+  }
   rcwa_status st = c.reserve(chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra);
   if (st) return st;
 

@@ -25,6 +25,7 @@ struct Ctx {
   void* split_ws = nullptr;  // operand slices of the split-precision GEMM
   size_t split_cap = 0;
   size_t arena_cap = 0, arena_off = 0;
+  int cus = 256;          // compute units of the device
   int eig_maxit = 30;
   long long qr_log = 0;  // QR rotation-log capacity per matrix (0 = default)
   int max_chunk = 0;

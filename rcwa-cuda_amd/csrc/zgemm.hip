@@ -42,14 +42,20 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 
 // Two workgroups per CU (their LDS fits twice): registers capped at 256 per
 // lane, which the three accumulator sets still fit without spilling.
+// tri: ZGEMM_TRI_C (only the tiles of C that meet its upper triangle are
+// formed; the others are left as they are) and ZGEMM_TRI_B (B is upper
+// triangular: the tile of columns n0..n0+BN-1 needs rows k < n0 + BN only).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void zgemm_kernel(
-    int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int tiles_n) {
+    int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int tiles_n,
+    int tri) {
   __shared__ zc As[2][BK][BM + PADA];
   __shared__ zc Bs[2][BK][BN + PADB];
   const int b = blockIdx.z;
   const int tn = blockIdx.x % tiles_n;
   const int tm = blockIdx.x / tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
+  if ((tri & ZGEMM_TRI_C) && m0 >= n0 + BN) return;  // uniform over the workgroup
+  if (tri & ZGEMM_TRI_B) K = min(K, n0 + BN);
   const zc* Ap = A.p + (long long)b * A.stride;
   const zc* Bp = B.p + (long long)b * B.stride;
   const int tid = threadIdx.x;
@@ -169,12 +175,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
 
 }  // namespace
 
-int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
-                  ZCMat D, ZMat C, int batch) {
+int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                      ZCMat D, ZMat C, int batch, int tri) {
   if (M <= 0 || N <= 0 || batch <= 0) return 0;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   dim3 grid(tiles_m * tiles_n, 1, batch);
   hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, s, M, N, K, alpha, A, B, beta, D, C,
-                     tiles_n);
+                     tiles_n, tri);
   return (int)hipGetLastError();
+}
+
+int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                  ZCMat D, ZMat C, int batch) {
+  return zgemm_batched_tri(s, M, N, K, alpha, A, B, beta, D, C, batch, 0);
 }
