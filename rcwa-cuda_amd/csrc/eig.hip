@@ -1418,8 +1418,9 @@ __global__ void trevc_init_kernel(int n, ZCMat Tm, ZMat Xm) {
 
 // Rows I0 .. I0+TB-1 of every column k > I0: one thread per column, its TB
 // unknowns in registers, the diagonal block of T in LDS (read by all lanes
-// at one address at a time).
-template <int TB>
+// at one address at a time).  UNIT: T has a unit diagonal and no shift is
+// taken (the columns of T^-1 by the same substitution: T u_k = e_k).
+template <int TB, bool UNIT>
 __global__ __launch_bounds__(256) void trevc_diag_kernel(int n, int I0, ZCMat Tm, ZMat Xm) {
   __shared__ zc tb[TB * TB];
   const int b = blockIdx.y, tid = threadIdx.x;
@@ -1434,7 +1435,7 @@ __global__ __launch_bounds__(256) void trevc_diag_kernel(int n, int I0, ZCMat Tm
   __syncthreads();
   const int k = I0 + 1 + blockIdx.x * 256 + tid;
   if (k >= n) return;
-  const zc lk = T[k + (long long)k * ld];
+  const zc lk = UNIT ? zmake(0, 0) : T[k + (long long)k * ld];
   const double smlnum = DBL_MIN * ((double)n / DBL_EPSILON);
   const double smin = fmax(DBL_EPSILON * zabs1(lk), smlnum);
   const int iend = min(k, I0 + nbk) - I0;  // unknown rows I0 .. I0+iend-1
@@ -1445,9 +1446,12 @@ __global__ __launch_bounds__(256) void trevc_diag_kernel(int n, int I0, ZCMat Tm
 #pragma unroll
   for (int i = TB - 1; i >= 0; --i) {
     if (i < iend) {
-      zc d = zsub(tb[i + i * TB], lk);
-      if (zabs1(d) < smin) d = zmake(smin, 0.0);
-      const zc xi = zdiv(x[i], d);
+      zc xi = x[i];
+      if (!UNIT) {
+        zc d = zsub(tb[i + i * TB], lk);
+        if (zabs1(d) < smin) d = zmake(smin, 0.0);
+        xi = zdiv(xi, d);
+      }
       x[i] = xi;
 #pragma unroll
       for (int m = 0; m < i; ++m) x[m] = zsub(x[m], zmul(tb[m + i * TB], xi));
@@ -1456,6 +1460,19 @@ __global__ __launch_bounds__(256) void trevc_diag_kernel(int n, int I0, ZCMat Tm
 #pragma unroll
   for (int i = 0; i < TB; ++i)
     if (i < iend) xc[i] = x[i];
+}
+
+// out[b n + j] = (|X[:, j]|_2, 0)
+__global__ __launch_bounds__(256) void col_norms_kernel(int n, ZCMat Xm, zc* out) {
+  __shared__ double red[4];
+  const int j = blockIdx.x, b = blockIdx.y;
+  const zc* col = Xm.p + (long long)b * Xm.stride + (long long)j * Xm.ld;
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += zabs2(col[i]);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[(long long)b * n + j] = zmake(sqrt(red[0] + red[1] + red[2] + red[3]), 0);
 }
 
 __global__ void diag_kernel(int n, ZCMat Tm, zc* w) {
@@ -1571,7 +1588,7 @@ int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, in
   return (int)hipGetLastError();
 }
 
-int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch) {
+int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch, bool unit) {
   if (n <= 0 || batch <= 0) return 0;
   constexpr int TB = TREVC_TB;
   long long blocks = ((long long)n * n + 255) / 256;
@@ -1582,8 +1599,12 @@ int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch) {
   const zc mone = zmake(-1, 0), one = zmake(1, 0);
   for (int I0 = ((n - 1) / TB) * TB; I0 >= 0; I0 -= TB) {
     if (n - I0 - 1 > 0) {
-      hipLaunchKernelGGL((trevc_diag_kernel<TB>), dim3((n - I0 - 1 + 255) / 256, batch),
-                         dim3(256), 0, s, n, I0, T, X);
+      if (unit)
+        hipLaunchKernelGGL((trevc_diag_kernel<TB, true>), dim3((n - I0 - 1 + 255) / 256, batch),
+                           dim3(256), 0, s, n, I0, T, X);
+      else
+        hipLaunchKernelGGL((trevc_diag_kernel<TB, false>), dim3((n - I0 - 1 + 255) / 256, batch),
+                           dim3(256), 0, s, n, I0, T, X);
       if (int e = (int)hipGetLastError()) return e;
     }
     const int nbk = std::min(TB, n - I0);
@@ -1606,7 +1627,7 @@ size_t eig_workspace_bytes(int n, int batch) {
 }
 
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row, hipEvent_t* marks, long long log_cap) {
+                int maxit_per_row, hipEvent_t* marks, long long log_cap, bool inverse) {
   if (n <= 0 || batch <= 0) return 0;
   zc* Zp = (zc*)ws;
   zc* Xp = Zp + (size_t)batch * n * n;
@@ -1643,9 +1664,26 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
                              ZCMat{nullptr, 0, 0}, A, batch, ZGEMM_TRI_C)))
     return e;
   if (marks) (void)hipEventRecord(marks[2], s);
-  if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch))) return e;
+  if ((e = trevc_batched(s, n, ZCMat{A.p, A.ld, A.stride}, X, batch, false))) return e;
   hipLaunchKernelGGL(diag_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, s, n,
                      ZCMat{A.p, A.ld, A.stride}, w);
+  if (inverse) {
+    // W^-1 of the unit-norm eigenvectors W = Z X D^-1 (D = diag |X e_k|, Z
+    // unitary): W^-1 = D X^-1 Z^H.  X is unit upper triangular, so X^-1 comes
+    // from the same blocked substitution (into Z's buffer, free now), and Ap
+    // still holds Z^H.
+    zc* nrm = (zc*)hws;  // n per matrix (the Hessenberg workspace is free)
+    hipLaunchKernelGGL(col_norms_kernel, dim3(n, batch), dim3(256), 0, s, n,
+                       ZCMat{Xp, n, nn}, nrm);
+    if ((e = (int)hipGetLastError())) return e;
+    if ((e = trevc_batched(s, n, ZCMat{Xp, n, nn}, Z, batch, true))) return e;
+    if ((e = zgemm_batched_tri(s, n, n, n, one, ZCMat{Zp, n, nn}, ZCMat{Ap, n, nn}, zero,
+                               ZCMat{nullptr, 0, 0}, W, batch, ZGEMM_TRI_A)))
+      return e;
+    if ((e = zscale_rc(s, n, n, W, nrm, n, nullptr, 0, batch))) return e;
+    if (marks) (void)hipEventRecord(marks[3], s);
+    return 0;
+  }
   // W = Z X, X upper triangular (the eigenvectors of T)
   if ((e = zgemm_batched_tri(s, n, n, n, zmake(1, 0), ZCMat{Zp, n, (long long)n * n},
                              ZCMat{Xp, n, (long long)n * n}, zmake(0, 0), ZCMat{nullptr, 0, 0}, W,

@@ -10,8 +10,8 @@ int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B
 // Structure flags of zgemm_batched_tri: ZGEMM_TRI_C forms only the 64 x 64
 // tiles of C that meet its upper triangle (the tiles strictly below are not
 // written); ZGEMM_TRI_B takes B as upper triangular (its entries below the
-// diagonal are not read).
-constexpr int ZGEMM_TRI_C = 1, ZGEMM_TRI_B = 2;
+// diagonal are not read); ZGEMM_TRI_A likewise takes A as upper triangular.
+constexpr int ZGEMM_TRI_C = 1, ZGEMM_TRI_B = 2, ZGEMM_TRI_A = 4;
 int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
                       ZCMat D, ZMat C, int batch, int tri);
 
@@ -50,8 +50,12 @@ size_t eig_workspace_bytes(int n, int batch);
 // before the QR sweeps, before the eigenvector phase and at the end.
 // log_cap > 0 caps the QR rotation log per matrix (entries; default 2n^2):
 // when it fills, the sweep kernel applies it itself.  A tuning and test knob.
+// inverse: W receives the inverse of the unit-norm eigenvector matrix
+// instead (formed from the Schur vectors and the triangular eigenvectors of
+// the Schur form, without a general inverse).
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row, hipEvent_t* marks = nullptr, long long log_cap = 0);
+                int maxit_per_row, hipEvent_t* marks = nullptr, long long log_cap = 0,
+                bool inverse = false);
 // The three phases, exposed for tests.
 int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
 // Blocked (compact-WY panels + ZGEMM updates); ws holds hess_blocked_ws_bytes(n, batch).
@@ -66,7 +70,8 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, in
 // The same sweeps with every rotation applied directly in global memory.
 int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                              int maxit_per_row);
-int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch);
+// unit: T has a unit diagonal; X = T^-1 (no shift).
+int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch, bool unit = false);
 
 // rcwa_kernels.hip -----------------------------------------------------------
 // Cell-centre rasterisation of layer-sorted shapes (shapes first[l] ..

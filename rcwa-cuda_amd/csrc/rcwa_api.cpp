@@ -186,8 +186,10 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // layer, the other split products together by ~3e-7 (~1e-6 at 21x21).
   CK(zgemm_batched(s, n, n, n, one, CM(b0), CM(b1), zero, none, M(b2), Bl));
   // eigendecomposition (rcwa.cpp:298-306)
+  // b3 <- W^-1 of the unit-norm eigenvectors (formed inside the eigensolver
+  // from the Schur vectors and the triangular eigenvectors of the Schur form)
   CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
-                 c.time_eig ? c.ev + 8 : nullptr, c.qr_log));
+                 c.time_eig ? c.ev + 8 : nullptr, c.qr_log, true));
   if (c.time_eig) {
     float th = 0, ts = 0, tv = 0;
     CKH(hipEventSynchronize(c.ev[11]));
@@ -207,11 +209,10 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // forms V = Q W diag(1/lambda) and inverts it; since P Q = W diag(lambda^2)
   // W^-1, V^-1 = diag(1/lambda) W^-1 P, so V^-1 V0 = diag(1/lambda) W^-1
   // (P V0): one product instead of a product and an inverse.  P V0 (V0 is
-  // a 2x2 block of diagonals) -> b2; W^-1 in place of W; Y = W^-1 P V0 ->
-  // b0 (P is not needed after this), FP64 in both precision forms (it
-  // carries the eigenvectors); A -> b0, B -> b1.
+  // a 2x2 block of diagonals) -> b2; W^-1 is in b3; Y = W^-1 P V0 -> b0 (P
+  // is not needed after this), FP64 in both precision forms (it carries the
+  // eigenvectors); A -> b0, B -> b1.
   CK(mul_v0(s, N, CM(b0), v0, M(b2), Bl));
-  CK(gj_inverse_batched(s, n, M(b3), Bl, w.gj, info_sing));
   CK(zgemm_batched(s, n, n, n, one, CM(b3), CM(b2), zero, none, M(b0), Bl));
   CK(build_ab(s, n, CM(b3), M(b0), ilam, M(b1), Bl));
   // A^-1

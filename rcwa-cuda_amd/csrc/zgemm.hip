@@ -43,8 +43,10 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 // Two workgroups per CU (their LDS fits twice): registers capped at 256 per
 // lane, which the three accumulator sets still fit without spilling.
 // tri: ZGEMM_TRI_C (only the tiles of C that meet its upper triangle are
-// formed; the others are left as they are) and ZGEMM_TRI_B (B is upper
-// triangular: the tile of columns n0..n0+BN-1 needs rows k < n0 + BN only).
+// formed; the others are left as they are), ZGEMM_TRI_B (B is upper
+// triangular: the tile of columns n0..n0+BN-1 needs rows k < n0 + BN only)
+// and ZGEMM_TRI_A (A is upper triangular: the tile of rows m0.. needs
+// columns k >= m0 only).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void zgemm_kernel(
     int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int tiles_n,
     int tri) {
@@ -56,6 +58,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
   const int m0 = tm * BM, n0 = tn * BN;
   if ((tri & ZGEMM_TRI_C) && m0 >= n0 + BN) return;  // uniform over the workgroup
   if (tri & ZGEMM_TRI_B) K = min(K, n0 + BN);
+  const int kbeg = (tri & ZGEMM_TRI_A) ? min(m0, K) : 0;  // a multiple of BK
   const zc* Ap = A.p + (long long)b * A.stride;
   const zc* Bp = B.p + (long long)b * B.stride;
   const int tid = threadIdx.x;
@@ -115,13 +118,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
     }
   };
 
-  const int nk = (K + BK - 1) / BK;
-  gload(0);
+  const int nk = (K - kbeg + BK - 1) / BK;
+  gload(kbeg);
   sstore(0);
   __syncthreads();
   for (int t = 0; t < nk; ++t) {
     const int cur = t & 1;
-    if (t + 1 < nk) gload((t + 1) * BK);
+    if (t + 1 < nk) gload(kbeg + (t + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       zc a[2], bb[2];
