@@ -40,9 +40,9 @@ sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
 METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
 # HBM bytes (read + written) of the QR phase per 882x882 layer matrix, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR phase's kernels at
-# 240 matrices per launch, scaled by the counters' reading on a copy of known
-# size (profiles/qr_phase_21x21_pmc_traffic.md).
-QR_PHASE_HBM_BYTES_PER_MATRIX = 3063.2e9 / 240
+# 510 matrices per launch, scaled by the counters' reading on a copy of known
+# size (--hbm-copy-probe; profiles/qr_phase_21x21_pmc_traffic.md).
+QR_PHASE_HBM_BYTES_PER_MATRIX = 6727.9e9 / 510
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -83,6 +83,12 @@ class Hip:
     def upload(self, dst, arr):
         rc = self.lib.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(arr.ctypes.data),
                                 ctypes.c_size_t(arr.nbytes), 1)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+    def copy(self, dst, src, nbytes):
+        rc = self.lib.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src),
+                                ctypes.c_size_t(nbytes), 3)  # device to device
         if rc != 0:
             raise RuntimeError("hipMemcpy failed: %d" % rc)
 
@@ -198,9 +204,22 @@ def main():
     ap.add_argument("--serve", type=int, default=0,
                     help="also run the serving form on this many requests of the stream")
     ap.add_argument("--cpu-sample", choices=["layer", "star"], help=argparse.SUPPRESS)
+    ap.add_argument("--hbm-copy-probe", action="store_true",
+                    help="only copy 1 GiB between two device buffers three times (the runtime's "
+                         "copy kernel): the known-size reference for HBM traffic counters")
     args = ap.parse_args()
     if args.cpu_sample:
         cpu_sample(args.cpu_sample)
+        return
+    if args.hbm_copy_probe:
+        hip = Hip()
+        a, b = hip.malloc(1 << 30), hip.malloc(1 << 30)
+        for _ in range(3):
+            hip.copy(b, a, 1 << 30)
+        hip.sync()
+        hip.free(a)
+        hip.free(b)
+        print(json.dumps({"hbm_copy_probe_bytes": 1 << 30, "copies": 3}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the multi-GPU form of this path is design-only "
