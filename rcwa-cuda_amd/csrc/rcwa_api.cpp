@@ -111,6 +111,20 @@ rcwa_status Ctx::reserve(size_t bytes) {
   return RCWA_OK;
 }
 
+rcwa_status Ctx::set_precision(rcwa_precision p) {
+  precision = p;
+  // The split form's slice workspace lies outside the arena and its budget:
+  // an FP64 solve releases it, so that the arena can grow into that HBM.
+  if (p == RCWA_FP64 && split_ws) {
+    CKH(hipSetDevice(device));
+    CKH(hipStreamSynchronize(stream));
+    CKH(hipFree(split_ws));
+    split_ws = nullptr;
+    split_cap = 0;
+  }
+  return RCWA_OK;
+}
+
 void* Ctx::take(size_t bytes) {
   arena_off = (arena_off + 255) & ~(size_t)255;
   void* p = arena + arena_off;
@@ -789,7 +803,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
 }
 
 rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_precision prec) {
-  c.precision = prec;
+  if (rcwa_status st = c.set_precision(prec)) return st;
   c.last_cols.clear();
   std::vector<Job> jobs;
   for (int p = 0; p < nprob; ++p) {
@@ -869,7 +883,7 @@ rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_s
                         rcwa_precision prec, rcwa_results* out) {
   if (npoints < 0 || (npoints > 0 && (!src || !eps || !out)) || nsrc <= 0)
     return rcwa_set_error(RCWA_E_ARG, "bad sweep arguments");
-  c.precision = prec;
+  if (rcwa_status st = c.set_precision(prec)) return st;
   c.last_cols.clear();
   const int M = plan.M, L = plan.dev.num_layers;
   // rows: points with bitwise equal eps (and mu) share one, numbered in

@@ -40,6 +40,8 @@ struct Ctx {
   void release();
   rcwa_status reserve(size_t bytes);  // arena of at least `bytes`; resets the bump pointer
   void* take(size_t bytes);
+  // Sets the precision of the next solves; FP64 frees the split workspace.
+  rcwa_status set_precision(rcwa_precision p);
   // Error if the buffers taken since reserve() overran the arena; called after
   // carving, before any kernel sees the buffers.
   rcwa_status arena_check() const;
