@@ -1,18 +1,21 @@
 // Batched eigendecomposition of general complex matrices on gfx950: the
 // replacement of Eigen::ComplexEigenSolver at rcwa.cpp:298-306.
 //
-// Three phases, each one batched launch:
+// Three phases, each batched over the matrices:
 //   1. Householder reduction to upper Hessenberg form H = Q^H A Q, Q
-//      accumulated (one workgroup per matrix);
+//      accumulated (blocked: compact-WY panels in hess.hip, updates on the
+//      matrix cores; the unblocked one-workgroup form for n > 4096);
 //   2. complex Schur form T = Z^H A Z by implicitly shifted single-shift QR
 //      sweeps with Wilkinson shifts (exceptional shifts at the 10th and 20th
-//      sweep on one block) and deflation on negligible subdiagonals (one
-//      workgroup per matrix; every Givens rotation is applied by all threads,
-//      rows then columns);
+//      sweep on one block) and deflation on negligible subdiagonals: one
+//      workgroup per matrix sweeps the active block and logs its rotations,
+//      qr_apply_kernel applies the log to the rows of Z, and T is formed
+//      afterwards as Z^H (A Z) on the matrix cores;
 //   3. eigenvectors: the unit upper-triangular X with T X = X diag(T) by
-//      back-substitution, one wave per eigenvector (tiny pivots are replaced
-//      by max(eps |t_kk|, safe minimum), as in LAPACK's xTREVC), then
-//      W = Z X on the matrix cores and unit 2-norm columns.
+//      blocked back-substitution (tiny pivots are replaced by
+//      max(eps |t_kk|, safe minimum), as in LAPACK's xTREVC), then W = Z X on
+//      the matrix cores and unit 2-norm columns, or, for the layer stage,
+//      W^-1 = D X^-1 Z^H directly.
 // The S-matrices the path builds from W depend only on the spans of
 // eigenvectors and not on their scale or order, so W need not match Eigen's
 // normalisation or sort order.
