@@ -176,3 +176,23 @@ def test_amplitudes_vs_oracle(ctx, seed, lossless):
             assert np.abs(g[4][j] - t_ref).max() < TOL
             assert np.abs(s[4][j] - r_ref).max() < TOL
             assert np.abs(s[5][j] - t_ref).max() < TOL
+
+
+def test_layer_batches_split_at_two_per_cu(ctx):
+    """More layers of n >= 256 than two per CU (13x13 harmonics, n = 338;
+    35 structures x 15 layers + one of 2 = 527 > 2 x 256) are split into
+    equal layer batches, with the fold of the first running beside the
+    second: every result equals the solve of its structure in a chunk small
+    enough for one batch, and the two-layer structure equals the oracle."""
+    rng = np.random.default_rng(47)
+    probs = [W.random_small(rng, hx=6, hy=6, L=2, grid=16)]
+    probs += [W.random_small(rng, hx=6, hy=6, L=15, grid=16) for _ in range(35)]
+    together = ctx.solve(probs)
+    ctx.set_chunk(12)  # 180 layers per chunk: one batch
+    try:
+        apart = ctx.solve(probs)
+    finally:
+        ctx.set_chunk(0)
+    for g, h in zip(together, apart):
+        assert max(_cmp(g, h)) < 1e-12
+    assert max(_cmp(together[0], O.solve(probs[0]))) < TOL
