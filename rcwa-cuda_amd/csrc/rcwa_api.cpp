@@ -80,6 +80,9 @@ void Ctx::release() {
   if (split_ws) (void)hipFree(split_ws);
   split_ws = nullptr;
   split_cap = 0;
+  if (split_ws2) (void)hipFree(split_ws2);
+  split_ws2 = nullptr;
+  split_cap2 = 0;
   if (arena) (void)hipFree(arena);
   arena = nullptr;
   arena_cap = 0;
@@ -115,12 +118,14 @@ rcwa_status Ctx::set_precision(rcwa_precision p) {
   precision = p;
   // The split form's slice workspace lies outside the arena and its budget:
   // an FP64 solve releases it, so that the arena can grow into that HBM.
-  if (p == RCWA_FP64 && split_ws) {
+  if (p == RCWA_FP64 && (split_ws || split_ws2)) {
     CKH(hipSetDevice(device));
     CKH(hipStreamSynchronize(stream));
-    CKH(hipFree(split_ws));
-    split_ws = nullptr;
-    split_cap = 0;
+    if (stream2) CKH(hipStreamSynchronize(stream2));
+    if (split_ws) CKH(hipFree(split_ws));
+    if (split_ws2) CKH(hipFree(split_ws2));
+    split_ws = split_ws2 = nullptr;
+    split_cap = split_cap2 = 0;
   }
   return RCWA_OK;
 }
@@ -139,22 +144,37 @@ rcwa_status Ctx::arena_check() const {
 }
 
 // ------------------------------------------------------------------ GEMM dispatch
+namespace {
+// A product in the context's precision on stream st; the split form's slices
+// go to ws (grown on demand, after st has drained).
+int gemm_on(rcwa_precision prec, hipStream_t st, void*& ws, size_t& cap, int M, int N, int K,
+            zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch) {
+  if (prec == RCWA_SPLIT) {
+    const size_t need = zgemm_split_ws_bytes(M, N, K, batch);
+    if (need > cap) {
+      hipError_t e = hipStreamSynchronize(st);
+      if (e == hipSuccess && ws) e = hipFree(ws);
+      ws = nullptr;
+      cap = 0;
+      if (e == hipSuccess) e = hipMalloc(&ws, need);
+      if (e != hipSuccess) return (int)e;
+      cap = need;
+    }
+    return zgemm_split_batched_ws(st, M, N, K, alpha, A, B, beta, D, C, batch, ws);
+  }
+  return zgemm_batched(st, M, N, K, alpha, A, B, beta, D, C, batch);
+}
+}  // namespace
+
 int Ctx::gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C,
               int batch) {
-  if (precision == RCWA_SPLIT) {
-    const size_t need = zgemm_split_ws_bytes(M, N, K, batch);
-    if (need > split_cap) {
-      hipError_t e = hipStreamSynchronize(stream);
-      if (e == hipSuccess && split_ws) e = hipFree(split_ws);
-      split_ws = nullptr;
-      split_cap = 0;
-      if (e == hipSuccess) e = hipMalloc(&split_ws, need);
-      if (e != hipSuccess) return (int)e;
-      split_cap = need;
-    }
-    return zgemm_split_batched_ws(stream, M, N, K, alpha, A, B, beta, D, C, batch, split_ws);
-  }
-  return zgemm_batched(stream, M, N, K, alpha, A, B, beta, D, C, batch);
+  return gemm_on(precision, stream, split_ws, split_cap, M, N, K, alpha, A, B, beta, D, C, batch);
+}
+
+int Ctx::gemm2(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C,
+               int batch) {
+  return gemm_on(precision, stream2, split_ws2, split_cap2, M, N, K, alpha, A, B, beta, D, C,
+                 batch);
 }
 
 // ------------------------------------------------------------------ layer stage
@@ -610,12 +630,12 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     CK(convmat_gather(s, F, Nx, Ny, hx, hy, ZMat{urc, N, N2}, L, map));
   }
   CKH(hipEventRecord(c.ev[1], s));
-  // ---- fold (main.cpp:143-170), on its own stream in the FP64 form: fold
-  // step l needs only layer l's S-matrices, so it runs as soon as the layer
-  // batch holding them is done, beside the next batch's layer stage (the
-  // layer-major order puts layer l of every job in one batch or two).  The
-  // split form keeps one stream: its products share one slice workspace.
-  const bool overlap = c.precision == RCWA_FP64 && c.stream2 != nullptr;
+  // ---- fold (main.cpp:143-170), on its own stream: fold step l needs only
+  // layer l's S-matrices, so it runs as soon as the layer batch holding them
+  // is done, beside the next batch's layer stage (the layer-major order puts
+  // layer l of every job in one batch or two).  In the split form the fold's
+  // products use their own slice workspace (Ctx::gemm2).
+  const bool overlap = c.stream2 != nullptr;
   hipStream_t fs = overlap ? c.stream2 : s;
   std::vector<hipEvent_t> evs;  // evs[0]: inputs uploaded; then one per layer batch
   struct EventsGuard {
@@ -635,7 +655,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   };
   auto fgemm = [&](int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D_,
                    ZMat C_, int bt) {
-    return overlap ? zgemm_batched(fs, M_, N_, K_, al, A_, B_, be, D_, C_, bt)
+    return overlap ? c.gemm2(M_, N_, K_, al, A_, B_, be, D_, C_, bt)
                    : c.gemm(M_, N_, K_, al, A_, B_, be, D_, C_, bt);
   };
   if ((st = fence(s, fs)) != RCWA_OK) return st;

@@ -24,6 +24,8 @@ struct Ctx {
   char* arena = nullptr;
   void* split_ws = nullptr;  // operand slices of the split-precision GEMM
   size_t split_cap = 0;
+  void* split_ws2 = nullptr;  // the same for the fold's products on stream2
+  size_t split_cap2 = 0;
   size_t arena_cap = 0, arena_off = 0;
   int cus = 256;          // compute units of the device
   int eig_maxit = 30;
@@ -46,6 +48,9 @@ struct Ctx {
   // carving, before any kernel sees the buffers.
   rcwa_status arena_check() const;
   int gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch);
+  // The same on stream2 (the fold beside the layer stage), with its own split workspace.
+  int gemm2(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C,
+            int batch);
 };
 
 // Work buffers of the layer stage for a sub-batch of Bl layers.
