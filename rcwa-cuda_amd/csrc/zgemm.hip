@@ -34,6 +34,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int BM = 64, BN = 64, BK = 16;
+constexpr unsigned XCDS = 8;  // accelerator complex dies of the MI355X, each with its own L2
 constexpr int PADA = 1, PADB = 1;
 
 __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
@@ -52,9 +53,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void z
     int tri) {
   __shared__ zc As[2][BK][BM + PADA];
   __shared__ zc Bs[2][BK][BN + PADB];
-  const int b = blockIdx.z;
-  const int tn = blockIdx.x % tiles_n;
-  const int tm = blockIdx.x / tiles_n;
+  // One-dimensional grid of tiles x batch.  Workgroups are dispatched to the
+  // XCDs round-robin by their index; the index is remapped so that each XCD
+  // receives a contiguous range of tiles (the tiles of one matrix together),
+  // whose A rows and B columns then meet in that XCD's L2.
+  const unsigned total = gridDim.x, L = blockIdx.x;
+  const unsigned per = total / XCDS, rem = total % XCDS, xcd = L % XCDS, idx = L / XCDS;
+  const unsigned v = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+  const int tiles = ((M + BM - 1) / BM) * tiles_n;  // tiles per matrix
+  const int b = (int)(v / (unsigned)tiles);
+  const int t = (int)(v % (unsigned)tiles);
+  const int tn = t % tiles_n;
+  const int tm = t / tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
   if ((tri & ZGEMM_TRI_C) && m0 >= n0 + BN) return;  // uniform over the workgroup
   if (tri & ZGEMM_TRI_B) K = min(K, n0 + BN);
@@ -182,9 +192,10 @@ int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCM
                       ZCMat D, ZMat C, int batch, int tri) {
   if (M <= 0 || N <= 0 || batch <= 0) return 0;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  dim3 grid(tiles_m * tiles_n, 1, batch);
-  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, s, M, N, K, alpha, A, B, beta, D, C,
-                     tiles_n, tri);
+  const long long total = (long long)tiles_m * tiles_n * batch;
+  if (total > 0x7fffffffLL) return (int)hipErrorInvalidValue;
+  hipLaunchKernelGGL(zgemm_kernel, dim3((unsigned)total, 1, 1), dim3(256), 0, s, M, N, K, alpha, A,
+                     B, beta, D, C, tiles_n, tri);
   return (int)hipGetLastError();
 }
 
