@@ -115,6 +115,24 @@ def test_split_form_error_vs_fp64(ctx):
     assert dR < 1e-6 and dT < 1e-6
 
 
+def test_precision_switches_on_one_context(ctx):
+    """FP64, split, FP64, split on one context: the split form's slice
+    workspaces (layer stage and fold stream) are allocated on demand and
+    released by the next FP64 solve; every repeat gives the same results."""
+    rng = np.random.default_rng(53)
+    probs = [W.random_small(rng, hx=3, hy=3, L=4, grid=16) for _ in range(3)]
+    f1 = ctx.solve(probs)
+    s1 = ctx.solve(probs, precision=G.RCWA_SPLIT)
+    f2 = ctx.solve(probs)
+    s2 = ctx.solve(probs, precision=G.RCWA_SPLIT)
+    for a, b in zip(f1, f2):
+        assert max(_cmp(a, b)) == 0.0
+    for a, b in zip(s1, s2):
+        assert max(_cmp(a, b)) == 0.0
+    for a, b in zip(f1, s1):
+        assert max(_cmp(a, b)) < 1e-6
+
+
 def test_solve_batch_mixed_layer_counts(ctx):
     """One chunk with different layer counts: the fold runs on a shrinking
     prefix of the jobs (layer-major layout)."""
