@@ -214,3 +214,20 @@ def test_layer_batches_split_at_two_per_cu(ctx):
     for g, h in zip(together, apart):
         assert max(_cmp(g, h)) < 1e-12
     assert max(_cmp(together[0], O.solve(probs[0]))) < TOL
+
+
+def test_c2_full_size_lossless_energy(ctx):
+    """The benchmark's structure at full size (21x21 harmonics, n = 882, 20
+    layers on 512^2 grids) at two wavelengths of its sweep, every permittivity
+    made real: R + T = 1 to 1e-10 for TE and TM, and no negative order.  A
+    size-independent check at the size where the CPU oracle would take about a
+    quarter of an hour per solve."""
+    probs = W.c2_sweep(indices=[0, 137])
+    for p in probs:
+        p.er = p.er.real.astype(np.complex128)
+    out = ctx.solve(probs)
+    for R, T, tot in out:
+        dev = np.abs(np.asarray(tot).sum(axis=1) - 1.0).max()
+        print("R + T - 1: %.3e, min order %.3e" % (dev, min(R.min(), T.min())))
+        assert R.min() > -1e-12 and T.min() > -1e-12
+        assert dev < 1e-10
