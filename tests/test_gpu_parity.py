@@ -2,6 +2,8 @@
 
 Tolerances (from the FP64 contract): per-order and total R/T within 1e-9
 absolute of the oracle; R + T = 1 within 1e-10 for lossless stacks."""
+import math
+
 import numpy as np
 import pytest
 
@@ -231,3 +233,26 @@ def test_c2_full_size_lossless_energy(ctx):
         print("R + T - 1: %.3e, min order %.3e" % (dev, min(R.min(), T.min())))
         assert R.min() > -1e-12 and T.min() > -1e-12
         assert dev < 1e-10
+
+
+def test_c2_full_size_split_and_sweep_plan(ctx):
+    """Full size again (21x21 harmonics, 20 layers), two wavelengths: the
+    split-precision form stays within the 1e-6 target of the FP64 form, and
+    the sweep plan (convolution matrices mixed from material masks) equals the
+    direct solve on the rasterised grids to 1e-11."""
+    idx = [20, 150]
+    probs = W.c2_sweep(indices=idx)
+    f = ctx.solve(probs)
+    s = ctx.solve(probs, precision=G.RCWA_SPLIT)
+    for a, b in zip(f, s):
+        assert max(_cmp(a, b)) < 1e-6
+    material, thick, lams, eps = W.c2_structure()
+    th = math.radians(65.0)
+    srcs = [[(lams[i], th, 0.0, 1.0, 0.0), (lams[i], th, 0.0, 0.0, 1.0)] for i in idx]
+    sw = ctx.sweep(material, thick, 120.0, 120.0, eps.shape[1], 10, 10)
+    try:
+        pres = sw.run(srcs, eps[idx])
+    finally:
+        sw.close()
+    for a, b in zip(f, pres):
+        assert max(_cmp(a, b)) < 1e-11
