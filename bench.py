@@ -11,11 +11,17 @@ The JSON line reports:
                completes with the step)
   roofline     the QR phase of the eigensolver (sweep kernel, log apply to
                the Schur vectors, the two products forming the Schur form;
-               the dominant phase), its algorithmic FP64 flops per launch
-               over its event-timed average duration, against the FP64 peak
-  cpu_baseline the CPU FP64 oracle on a bounded sample (one layer S-matrix
-               and one Redheffer product of the same workload, one thread
-               each), extrapolated to a full solve
+               the dominant phase), whose bound is HBM traffic: its
+               compulsory bytes per launch over its event-timed average
+               duration, against the HBM peak, beside the traffic measured
+               with PMC counters and its FP64 flop rate
+  split_sample the split-precision form on 8 of the timed wavelengths:
+               max |dR|, |dT| against the FP64 results of this run
+  cpu_baseline the CPU FP64 oracle on a bounded sample of the same
+               workload, run after the GPU phases as one single-threaded
+               worker process per host core of the run's CPU share (half
+               time one layer S-matrix, half one Redheffer product),
+               extrapolated to a full solve: per-core and all-core solves/s
   split        (--split) the split-precision form on the same wavelengths:
                solves/s and max |dR|, |dT| against this FP64 run
   planner      (--planner) the same wavelengths through a sweep plan (the
@@ -44,6 +50,57 @@ METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| 
 # size (--hbm-copy-probe; profiles/qr_phase_21x21_pmc_traffic.md).
 QR_PHASE_HBM_BYTES_PER_MATRIX = 6727.9e9 / 510
 FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBPS = 8000.0
+
+
+def cpu_workers():
+    """Host cores of this run's CPU share: the affinity mask, capped at the
+    16 cores a one-GPU box provides."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return "unknown"
+
+
+def cpu_baseline():
+    """One single-threaded oracle process per core of the CPU share, all at
+    once: even workers time one SMatrixLayer, odd ones one RedhefferProduct
+    (N = 441, the sweep's size).  A solve is 20 layers + 22 star products."""
+    W = cpu_workers()
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1")
+    kinds = ["layer" if k % 2 == 0 else "star" for k in range(W)] if W > 1 else ["both"]
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-sample", k],
+                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, text=True)
+             for k in kinds]
+    t = {"layer": [], "star": []}
+    for pr in procs:
+        o, e = pr.communicate(timeout=1800)
+        if pr.returncode != 0:
+            raise RuntimeError("CPU baseline worker failed: " + e[-500:])
+        for line in o.strip().splitlines():
+            d = json.loads(line)
+            t[d["which"]].append(d["seconds"])
+    tl, ts = float(np.mean(t["layer"])), float(np.mean(t["star"]))
+    per_solve = 20 * tl + 22 * ts
+    return {"value": W / per_solve, "unit": "solves/s", "cores": W, "kind": "port",
+            "per_core_solves_per_s": 1.0 / per_solve, "cpu_model": cpu_model(),
+            "sample": "CPU FP64 oracle (naive-loop restatement of the reference's Eigen path), "
+                      "%d single-threaded workers at once on %d cores: %d timed one SMatrixLayer "
+                      "(mean %.1f s), %d one RedhefferProduct (mean %.1f s) at N=441; "
+                      "extrapolated to 20 layers + 22 star products per solve, x %d cores"
+                      % (W, W, len(t["layer"]), tl, len(t["star"]), ts, W)}
 
 
 def cpu_sample(which):
@@ -59,12 +116,14 @@ def cpu_sample(which):
     t0 = time.perf_counter()
     S11, S12 = O.smatrix_layer(erc, urc, Kx, Ky, 2 * math.pi / p.lam0, p.thick[10])
     t1 = time.perf_counter()
-    if which == "star":
+    if which in ("layer", "both"):
+        print(json.dumps({"which": "layer", "seconds": t1 - t0}), flush=True)
+    if which in ("star", "both"):
         S = (S11, S12, S12, S11)
         t0 = time.perf_counter()
         O.redheffer(S, S)
         t1 = time.perf_counter()
-    print(json.dumps({"which": which, "seconds": t1 - t0}))
+        print(json.dumps({"which": "star", "seconds": t1 - t0}), flush=True)
 
 
 class Hip:
@@ -203,7 +262,7 @@ def main():
                     help="requests of the stream timed one per call")
     ap.add_argument("--serve", type=int, default=0,
                     help="also run the serving form on this many requests of the stream")
-    ap.add_argument("--cpu-sample", choices=["layer", "star"], help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-sample", choices=["layer", "star", "both"], help=argparse.SUPPRESS)
     ap.add_argument("--hbm-copy-probe", action="store_true",
                     help="only copy 1 GiB between two device buffers three times (the runtime's "
                          "copy kernel): the known-size reference for HBM traffic counters")
@@ -231,15 +290,6 @@ def main():
     import workloads as W
 
     nlam_needed = (args.warmup + args.steps) * args.lams
-
-    # CPU baseline: two single-threaded oracle processes, concurrent with the GPU run
-    cpu_procs = []
-    if not args.no_cpu_baseline:
-        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1")
-        for which in ("layer", "star"):
-            cpu_procs.append(subprocess.Popen([sys.executable, os.path.abspath(__file__),
-                                               "--cpu-sample", which], stdout=subprocess.PIPE,
-                                              stderr=subprocess.PIPE, env=env, text=True))
 
     # wavelengths spread over the 200-point sweep: warmup chunks first, then the
     # timed chunks; a run of more than 200 solves revisits wavelengths.  One
@@ -301,16 +351,24 @@ def main():
     launches = max(st["n_eig_launches"], 1)
     per_launch_mats = st["n_eig_matrices"] / launches
     avg_ms = st["ms_eig_schur"] / launches
-    achieved = f_qr * per_launch_mats / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    tflops = f_qr * per_launch_mats / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    # compulsory HBM bytes per matrix: read the Hessenberg matrix, its Q and
+    # the copy of A; write the Schur form T and the Schur vectors Z
+    compulsory = 5.0 * n * n * 16
+    gbps = compulsory * per_launch_mats / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats
     roofline = {"kernel": "QR phase: schur_pipe_kernel + qr_apply_kernel + 2 x zgemm_kernel "
                           "(complex single-shift QR sweeps on the Hessenberg matrix, two "
                           "chained per pass; their rotations applied to the Schur vectors Z; "
                           "Schur form T = Z^H A Z)",
-                "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "traffic": QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats,
+                "bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": gbps / HBM_PEAK_GBPS, "traffic": traffic,
                 "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE passes, "
                                 "copy-kernel calibrated)",
+                "compulsory_bytes_per_launch": compulsory * per_launch_mats,
+                "traffic_over_compulsory": traffic / (compulsory * per_launch_mats),
+                "traffic_rate_gbps": traffic / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+                "fp64_tflops": tflops, "fp64_frac": tflops / FP64_PEAK_TFLOPS,
                 "avg_launch_ms": avg_ms, "matrices_per_launch": per_launch_mats}
     whole = st["flops_algorithmic"] / (st["ms_total"] * 1e-3) / 1e12 if st["ms_total"] else 0.0
 
@@ -332,6 +390,21 @@ def main():
                                         "ms_eig_schur", "ms_eig_vec", "ms_fold", "ms_out")},
         "roofline": roofline,
     }
+
+    # the split-precision form on 8 of the timed wavelengths, against this run's
+    # FP64 results (the metric's max |dR| vs FP64)
+    flat = [r for step in results for r in step]
+    pick = np.linspace(0, len(flat) - 1, min(8, len(flat))).round().astype(int).tolist()
+    sel = [probs[args.warmup * args.lams + i] for i in pick]
+    grids = [(er_devs[args.warmup * args.lams + i], ur_dev) for i in pick]
+    sout = ctx.solve(sel, G.RCWA_SPLIT, device_grids=grids, grid_shape=(512, 512))
+    out["split_sample"] = {
+        "wavelengths": len(pick),
+        "max_dR_vs_fp64": float(max(np.abs(flat[i][0] - o[0]).max() for i, o in zip(pick, sout))),
+        "max_dT_vs_fp64": float(max(np.abs(flat[i][1] - o[1]).max() for i, o in zip(pick, sout))),
+        "max_dtot_vs_fp64": float(max(np.abs(flat[i][2] - o[2]).max()
+                                      for i, o in zip(pick, sout)))}
+    progress("split-precision sample done")
 
     if args.split:
         sres = []
@@ -385,20 +458,9 @@ def main():
         progress("41x41-harmonic single structure")
         out["c5"] = c5_bench()
 
-    if cpu_procs:
-        t = {}
-        for pr in cpu_procs:
-            o, e = pr.communicate(timeout=1800)
-            if pr.returncode == 0:
-                d = json.loads(o.strip().splitlines()[-1])
-                t[d["which"]] = d["seconds"]
-        if len(t) == 2:
-            per_solve = 20 * t["layer"] + 22 * t["star"]
-            out["cpu_baseline"] = {
-                "value": 1.0 / per_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-                "sample": "CPU FP64 oracle, one thread: one SMatrixLayer (%.1f s) and one "
-                          "RedhefferProduct (%.1f s) at N=441, extrapolated to 20 layers + "
-                          "22 star products per solve" % (t["layer"], t["star"])}
+    if not args.no_cpu_baseline:
+        progress("CPU baseline: %d oracle workers" % cpu_workers())
+        out["cpu_baseline"] = cpu_baseline()
     ctx.close()
     print(json.dumps(out))
 

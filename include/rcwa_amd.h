@@ -37,8 +37,11 @@ typedef enum {
   RCWA_E_ARG = 1,         /* invalid argument or shape */
   RCWA_E_NOMEM = 2,       /* request exceeds the context's HBM budget */
   RCWA_E_EIG_NOCONV = 3,  /* QR iteration did not converge (rcwa.cpp:300 aborts here) */
-  RCWA_E_SINGULAR = 4,    /* an exactly singular matrix was inverted */
-  RCWA_E_NONFINITE = 5,   /* non-finite values in inputs or results */
+  RCWA_E_SINGULAR = 4,    /* an exactly singular matrix was inverted, or k_z = 0 in a
+                             half-space or the gap medium (the divisions of
+                             rcwa.cpp:225, 329, 541, 560 return inf/NaN there) */
+  RCWA_E_NONFINITE = 5,   /* non-finite values in inputs (grids, thicknesses, sources,
+                             media, sweep materials) or results */
   RCWA_E_HIP = 6,         /* HIP runtime error */
   RCWA_E_NOTREADY = 7,    /* server ticket not finished yet */
   RCWA_E_CANCELLED = 8    /* server ticket cancelled */
@@ -84,6 +87,14 @@ typedef struct {
   double R_tot, T_tot;
   rcwa_c128* r;
   rcwa_c128* t;
+  /* Output: conditioning of the solve's mode matrices, the largest over its
+   * layers of sqrt(2PQ) ||W^-1||_F for the unit-column eigenvector matrix W,
+   * an upper bound on the 2-norm condition number of W (1 for a layer whose
+   * eigenvectors are orthogonal; 0 when the device has no layers).  The
+   * reference has no such output (rcwa.cpp:298-306 takes W as it comes);
+   * a large value flags nearly defective layer matrices, whose R and T carry
+   * correspondingly less accuracy. */
+  double cond;
 } rcwa_results;
 
 /* One independent solve: a device, its media and nsrc sources.  Sources

@@ -36,6 +36,9 @@ int zscale_cols_to(hipStream_t s, int rows, int cols, ZCMat src, const zc* c, lo
                    ZMat dst, int batch);
 // Normalise every column to unit 2-norm.
 int znormalize_cols(hipStream_t s, int n, ZMat M, int batch);
+// out[b] = scale * ||M_b||_F (Frobenius norm) for a batch of rows x cols matrices.
+int zfrob_norms(hipStream_t s, int rows, int cols, ZCMat M, double scale, double* out,
+                int batch);
 // Count non-finite entries of `count` complex values (adds to *flag).
 int zcount_nonfinite(hipStream_t s, const zc* p, long long count, int* flag);
 

@@ -398,6 +398,27 @@ __global__ void nonfinite_kernel(const zc* p, long long count, int* flag) {
   if (bad) atomicAdd(flag, 1);
 }
 
+// out[b] = scale * ||M_b||_F for a batch of rows x cols matrices: one
+// workgroup per matrix, each thread a strided share of the columns' entries.
+__global__ __launch_bounds__(1024) void frob_kernel(int rows, int cols, ZCMat M, double scale,
+                                                    double* out) {
+  __shared__ double red[16];
+  const zc* p = M.p + (long long)blockIdx.x * M.stride;
+  double acc = 0;
+  for (int j = 0; j < cols; ++j) {
+    const zc* col = p + (long long)j * M.ld;
+    for (int i = threadIdx.x; i < rows; i += 1024) acc += zabs2(col[i]);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0;
+    for (int w = 0; w < 16; ++w) tot += red[w];
+    out[blockIdx.x] = scale * sqrt(tot);
+  }
+}
+
 inline dim3 grid_for(long long elems, int batch) {
   long long blocks = (elems + 255) / 256;
   if (blocks > 4096) blocks = 4096;
@@ -508,5 +529,12 @@ int znormalize_cols(hipStream_t s, int n, ZMat M, int batch) {
 
 int zcount_nonfinite(hipStream_t s, const zc* p, long long count, int* flag) {
   hipLaunchKernelGGL(nonfinite_kernel, grid_for(count, 1), dim3(256), 0, s, p, count, flag);
+  return (int)hipGetLastError();
+}
+
+int zfrob_norms(hipStream_t s, int rows, int cols, ZCMat M, double scale, double* out,
+                int batch) {
+  if (batch <= 0) return 0;
+  hipLaunchKernelGGL(frob_kernel, dim3(batch), dim3(1024), 0, s, rows, cols, M, scale, out);
   return (int)hipGetLastError();
 }

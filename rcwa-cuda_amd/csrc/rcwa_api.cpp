@@ -188,7 +188,7 @@ size_t layer_work_bytes(int N, int Bl) {
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
-                        ZCMat erci_pre, ZCMat urci_pre) {
+                        ZCMat erci_pre, ZCMat urci_pre, double* cond) {
   const int n = 2 * N;
   const long long n2 = (long long)n * n, N2 = (long long)N * N;
   hipStream_t s = c.stream;
@@ -237,6 +237,9 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
     c.stats.n_eig_launches += 1;
     c.stats.n_eig_matrices += Bl;
   }
+  // conditioning: W = Z X D^-1 has unit columns, so ||W||_F = sqrt(n) and
+  // cond_2(W) <= sqrt(n) ||W^-1||_F
+  if (cond) CK(zfrob_norms(s, n, n, CM(b3), std::sqrt((double)n), cond, Bl));
   // lambda, X (rcwa.cpp:317, 339)
   CK(layer_modes(s, n, lam2, lam, ilam, X, k0t, Bl));
   // A, B = W^-1 W0 +- V^-1 V0 (rcwa.cpp:326-338, W0 = I).  The reference
@@ -355,6 +358,45 @@ rcwa_status check_device(const rcwa_device* d, const rcwa_params* p) {
   if (p->Nx_harmonics < 0 || p->Ny_harmonics < 0)
     return rcwa_set_error(RCWA_E_ARG, "negative harmonic count");
   if (!(d->Lx > 0) || !(d->Ly > 0)) return rcwa_set_error(RCWA_E_ARG, "periods must be > 0");
+  if (!std::isfinite(d->Lx) || !std::isfinite(d->Ly))
+    return rcwa_set_error(RCWA_E_NONFINITE, "non-finite period");
+  for (int l = 0; l < d->num_layers; ++l)
+    if (!std::isfinite(d->t[l]) || d->t[l] < 0)
+      return rcwa_set_error(std::isfinite(d->t[l]) ? RCWA_E_ARG : RCWA_E_NONFINITE,
+                            "layer " + std::to_string(l) + ": thickness must be finite and >= 0");
+  for (rcwa_c128 z : {p->er_ref, p->ur_ref, p->er_trn, p->ur_trn})
+    if (!std::isfinite(z.re) || !std::isfinite(z.im))
+      return rcwa_set_error(RCWA_E_NONFINITE, "non-finite half-space permittivity/permeability");
+  return RCWA_OK;
+}
+
+// Sources must be finite; lambda0 > 0.
+rcwa_status check_source(const rcwa_source& sr) {
+  if (!std::isfinite(sr.lambda0) || !std::isfinite(sr.theta) || !std::isfinite(sr.phi) ||
+      !std::isfinite(sr.pte) || !std::isfinite(sr.ptm))
+    return rcwa_set_error(RCWA_E_NONFINITE, "non-finite source parameter");
+  if (!(sr.lambda0 > 0)) return rcwa_set_error(RCWA_E_ARG, "lambda0 must be > 0");
+  return RCWA_OK;
+}
+
+// k_z = 0 at some order in a half-space, or in the gap medium when the
+// device has layers: the reference divides by it (rcwa.cpp:225 in V0,
+// rcwa.cpp:329 through lambda = 0, rcwa.cpp:541 and 560 in the z field
+// components) and returns inf/NaN; here it is an error.
+rcwa_status check_kz(const rcwa_host::WaveVectors& wv, bool has_layers, int hx, int hy) {
+  const int N = (int)wv.Kx.size(), Q = 2 * hy + 1;
+  for (int m = 0; m < N; ++m) {
+    const cplx kz0 = 1.0 - wv.Kx[m] * wv.Kx[m] - wv.Ky[m] * wv.Ky[m];
+    const char* where = wv.Kz_ref[m] == 0.0   ? "reflection half-space"
+                        : wv.Kz_trn[m] == 0.0 ? "transmission half-space"
+                        : (has_layers && kz0 == 0.0) ? "gap medium"
+                                                     : nullptr;
+    if (where)
+      return rcwa_set_error(RCWA_E_SINGULAR,
+                            std::string("k_z = 0 in the ") + where + " at order (p, q) = (" +
+                                std::to_string(m / Q - hx) + ", " + std::to_string(m % Q - hy) +
+                                ") (Wood anomaly)");
+  }
   return RCWA_OK;
 }
 
@@ -376,7 +418,7 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
   z += 2 * T1 * n2;                                 // layer S11, S12
   z += 13 * (size_t)G * n2;                         // fold slots
   z += (size_t)G * 32 * N + (size_t)G * 4 * n;      // end diagonals, gathered columns
-  size_t bytes = z * sizeof(zc) + T1 * (sizeof(double) + 3 * sizeof(int)) + 64;
+  size_t bytes = z * sizeof(zc) + T1 * (2 * sizeof(double) + 3 * sizeof(int)) + 64;
   bytes += layer_work_bytes(N, Bl) + gj_workspace_bytes((int)n, std::max(G, 1)) +
            eig_workspace_bytes((int)n, Bl) + 64 * 256;
   return bytes;
@@ -467,6 +509,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   for (int g = 0; g < G; ++g) {
     Job& j = *jobs[g];
     j.wv = rcwa_host::wave_vectors(j.dev->Lx, j.dev->Ly, j.inc, hx, hy, j.media);
+    if ((st = check_kz(j.wv, j.dev->num_layers > 0, hx, hy)) != RCWA_OK) return st;
     std::vector<cplx> v0 = rcwa_host::gap_v0_diagonals(j.wv.Kx, j.wv.Ky);
     const double k0 = 2.0 * M_PI / j.inc.lambda0;
     for (int l = 0; l < j.dev->num_layers; ++l) {
@@ -511,6 +554,8 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   zc* dends = (zc*)c.take((size_t)G * 32 * N * sizeof(zc));
   zc* dcols = (zc*)c.take((size_t)G * 4 * n * sizeof(zc));
   int* dinfo = (int*)c.take(2 * T1 * sizeof(int));
+  double* dcond = (double*)c.take(T1 * sizeof(double));
+  int* dnonfin = (int*)c.take(sizeof(int));
   int* dcolidx = (int*)c.take(4 * sizeof(int));
   void* gjf = c.take(gj_workspace_bytes(n, std::max(G, 1)));
   LayerWork w;
@@ -536,6 +581,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     for (int l = 0; l < jobs[g]->dev->num_layers; ++l) jmap[(size_t)g * Lmax + l] = off[l] + g;
 
   CKH(hipMemsetAsync(dinfo, 0, 2 * T1 * sizeof(int), s));
+  CKH(hipMemsetAsync(dnonfin, 0, sizeof(int), s));
   int* info_eig = dinfo;
   int* info_sing = dinfo + T1;
   if (TL > 0) {
@@ -628,6 +674,21 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     CK(convmat_gather(s, F, Nx, Ny, hx, hy, ZMat{erc, N, N2}, L, map));
     CK(convmat_dft(s, gu, Nx, Ny, hx, hy, F, L, rowbuf));
     CK(convmat_gather(s, F, Nx, Ny, hx, hy, ZMat{urc, N, N2}, L, map));
+  }
+  // Non-finite grid values (or sweep materials) make every later stage
+  // non-finite, and the QR iteration then runs to its sweep limit: checked
+  // once here, on the convolution matrices, before the layer stage.
+  if (TL > 0) {
+    CK(zcount_nonfinite(s, erc, (long long)TL * N2, dnonfin));
+    CK(zcount_nonfinite(s, urc, (long long)TL * N2, dnonfin));
+    int hnf = 0;
+    CKH(hipMemcpyAsync(&hnf, dnonfin, sizeof(int), hipMemcpyDeviceToHost, s));
+    CKH(hipStreamSynchronize(s));
+    if (hnf) {
+      c.ev_on = false;
+      return rcwa_set_error(RCWA_E_NONFINITE, "non-finite permittivity or permeability in the "
+                                              "layer grids");
+    }
   }
   CKH(hipEventRecord(c.ev[1], s));
   // ---- fold (main.cpp:143-170), on its own stream: fold step l needs only
@@ -745,7 +806,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
                      dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
                      ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
                      ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
-                     info_eig + l0, info_sing + l0, erci_pre, urci_pre);
+                     info_eig + l0, info_sing + l0, erci_pre, urci_pre, dcond + l0);
     if (st) return st;
     if (progress) {
       CKH(hipStreamSynchronize(s));
@@ -774,6 +835,10 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   CKH(hipMemcpyAsync(hcols.data(), dcols, hcols.size() * sizeof(zc), hipMemcpyDeviceToHost, s));
   std::vector<int> hinfo(2 * T1);
   CKH(hipMemcpyAsync(hinfo.data(), dinfo, 2 * T1 * sizeof(int), hipMemcpyDeviceToHost, s));
+  std::vector<double> hcond(T1, 0.0);
+  if (TL > 0)
+    CKH(hipMemcpyAsync(hcond.data(), dcond, (size_t)TL * sizeof(double), hipMemcpyDeviceToHost,
+                       s));
   CKH(hipEventRecord(c.ev[6], s));
   CKH(hipStreamSynchronize(s));
   c.ev_on = false;
@@ -804,6 +869,10 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     const cplx* s21c = hcols.data() + (size_t)G * 2 * n + (size_t)g * 2 * n;
     j.cols.assign(s11c, s11c + 2 * n);
     j.cols.insert(j.cols.end(), s21c, s21c + 2 * n);
+    double cond = 0;
+    for (int l = 0; l < j.dev->num_layers; ++l) cond = std::max(cond, hcond[(size_t)off[l] + g]);
+    if (!std::isfinite(cond))
+      return rcwa_set_error(RCWA_E_NONFINITE, "non-finite eigenvector inverse");
     for (int si : j.srcs) {
       const rcwa_source& src = j.src[si];
       rcwa_host::Efficiencies e =
@@ -815,6 +884,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
       if (o.t) std::memcpy(o.t, e.t.data(), 3 * (size_t)N * sizeof(rcwa_c128));
       o.R_tot = e.R_tot;
       o.T_tot = e.T_tot;
+      o.cond = cond;
       if (!std::isfinite(e.R_tot) || !std::isfinite(e.T_tot))
         return rcwa_set_error(RCWA_E_NONFINITE, "non-finite efficiencies");
     }
@@ -834,7 +904,7 @@ rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_prec
     std::vector<size_t> mine;
     for (int si = 0; si < pr.nsrc; ++si) {
       const rcwa_source& sr = pr.src[si];
-      if (!(sr.lambda0 > 0)) return rcwa_set_error(RCWA_E_ARG, "lambda0 must be > 0");
+      if ((st = check_source(sr)) != RCWA_OK) return st;
       bool found = false;
       for (size_t k : mine) {
         Job& j = jobs[k];
@@ -906,6 +976,10 @@ rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_s
   if (rcwa_status st = c.set_precision(prec)) return st;
   c.last_cols.clear();
   const int M = plan.M, L = plan.dev.num_layers;
+  for (size_t k = 0; k < (size_t)npoints * M; ++k)
+    if (!std::isfinite(eps[k].re) || !std::isfinite(eps[k].im) ||
+        (mu && (!std::isfinite(mu[k].re) || !std::isfinite(mu[k].im))))
+      return rcwa_set_error(RCWA_E_NONFINITE, "non-finite material permittivity/permeability");
   // rows: points with bitwise equal eps (and mu) share one, numbered in
   // order of first appearance
   std::vector<int> row(npoints);
@@ -922,7 +996,7 @@ rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_s
     const size_t first = jobs.size();
     for (int si = 0; si < nsrc; ++si) {
       const rcwa_source& sr = ps[si];
-      if (!(sr.lambda0 > 0)) return rcwa_set_error(RCWA_E_ARG, "lambda0 must be > 0");
+      if (rcwa_status st = check_source(sr)) return st;
       bool found = false;
       for (size_t k = first; k < jobs.size(); ++k) {
         Job& j = jobs[k];

@@ -397,6 +397,7 @@ rcwa_status rcwa_efficiencies(const rcwa_source* src, const rcwa_params* p, cons
   if (out->t) std::memcpy(out->t, e.t.data(), 3 * (size_t)N * sizeof(rcwa_c128));
   out->R_tot = e.R_tot;
   out->T_tot = e.T_tot;
+  out->cond = 0;  // no layer matrices at this stage
   return RCWA_OK;
 }
 

@@ -73,12 +73,13 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
                    int G, int Bl);
 
 // erci_pre / urci_pre: [[eps]]^-1 / [[mu]]^-1 already formed (p != nullptr),
-// else the stage inverts erc / urc itself.
+// else the stage inverts erc / urc itself.  cond (optional, Bl doubles)
+// receives sqrt(n) ||W^-1||_F per layer, a bound on cond_2(W).
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
                         ZCMat erci_pre = ZCMat{nullptr, 0, 0},
-                        ZCMat urci_pre = ZCMat{nullptr, 0, 0});
+                        ZCMat urci_pre = ZCMat{nullptr, 0, 0}, double* cond = nullptr);
 
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
                          void* gjws, int* info_sing, bool keep_slots);

@@ -54,6 +54,7 @@ struct Request {
   double t_submit = 0, t_start = 0, t_finish = 0;
   int batch_size = 0;
   bool cancelled = false;
+  bool taken = false;  // admitted into a batch (set under the server lock)
 };
 
 }  // namespace
@@ -68,6 +69,7 @@ struct rcwa_server {
   std::vector<std::thread> workers;
   std::vector<std::unique_ptr<Ctx>> ctxs;
   bool stop = false;
+  int waiters = 0;  // threads blocked in rcwa_server_wait
   uint64_t next_ticket = 1;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   std::string init_error;
@@ -85,18 +87,14 @@ struct rcwa_server {
         std::unique_lock<std::mutex> lk(mu);
         cv_work.wait(lk, [&] { return stop || !pending.empty(); });
         if (stop && pending.empty()) return;
-        // oldest request decides the bucket
-        while (!pending.empty() && pending.front()->cancelled) pending.pop_front();
+        // oldest request decides the bucket (cancelled requests have already
+        // left the queue: rcwa_server_cancel removes them)
         if (pending.empty()) continue;
         const auto key = pending.front()->key;
         const int maxb = cfg.max_batch > 0 ? cfg.max_batch : 1 << 30;
         int TL = 0, Lmax = 0;
         for (auto it = pending.begin(); it != pending.end() && (int)batch.size() < maxb;) {
           Request* r = *it;
-          if (r->cancelled) {
-            it = pending.erase(it);
-            continue;
-          }
           if (r->key != key) {
             ++it;
             continue;
@@ -112,12 +110,18 @@ struct rcwa_server {
           batch.push_back(r);
           it = pending.erase(it);
         }
+        // stamped inside the admission critical section, so that cancel
+        // (which reads these under the same lock) never sees a request that
+        // is already in a batch as still queued
+        const double ts = now();
+        for (Request* r : batch) {
+          r->taken = true;
+          r->t_start = ts;
+        }
       }
-      const double ts = now();
       std::vector<ProblemRef> refs(batch.size());
       for (size_t i = 0; i < batch.size(); ++i) {
         Request* r = batch[i];
-        r->t_start = ts;
         refs[i].p = rcwa_problem{&r->dev, &r->par, r->src.data(), (int)r->src.size()};
         refs[i].out = r->res.data();
       }
@@ -156,6 +160,7 @@ rcwa_status fill(rcwa_server* s, Request* r, rcwa_results* out, rcwa_ticket_info
         std::memcpy(out[i].t, r->ta.data() + 3 * i * r->N, 3 * r->N * sizeof(rcwa_c128));
       out[i].R_tot = r->res[i].R_tot;
       out[i].T_tot = r->res[i].T_tot;
+      out[i].cond = r->res[i].cond;
     }
   }
   (void)s;
@@ -243,6 +248,9 @@ rcwa_status rcwa_server_submit(rcwa_server* s, const rcwa_problem* p, uint64_t* 
   return RCWA_OK;
 }
 
+// A ticket's entry is erased once poll or wait has returned its final status
+// (a result, an error or RCWA_E_CANCELLED): the server keeps no finished
+// requests, and a later call with the same ticket reports an unknown ticket.
 rcwa_status rcwa_server_poll(rcwa_server* s, uint64_t ticket, rcwa_results* out,
                              rcwa_ticket_info* info) {
   if (!s) return rcwa_set_error(RCWA_E_ARG, "null server");
@@ -250,9 +258,12 @@ rcwa_status rcwa_server_poll(rcwa_server* s, uint64_t ticket, rcwa_results* out,
   auto it = s->all.find(ticket);
   if (it == s->all.end()) return rcwa_set_error(RCWA_E_ARG, "unknown ticket");
   Request* r = it->second.get();
-  if (r->cancelled) return rcwa_set_error(RCWA_E_CANCELLED, "ticket cancelled");
   if (r->status == RCWA_E_NOTREADY) return RCWA_E_NOTREADY;
-  return fill(s, r, out, info);
+  rcwa_status st = r->status == RCWA_E_CANCELLED
+                       ? rcwa_set_error(RCWA_E_CANCELLED, "ticket cancelled")
+                       : fill(s, r, out, info);
+  s->all.erase(it);
+  return st;
 }
 
 rcwa_status rcwa_server_wait(rcwa_server* s, uint64_t ticket, rcwa_results* out,
@@ -261,36 +272,72 @@ rcwa_status rcwa_server_wait(rcwa_server* s, uint64_t ticket, rcwa_results* out,
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->all.find(ticket);
   if (it == s->all.end()) return rcwa_set_error(RCWA_E_ARG, "unknown ticket");
-  Request* r = it->second.get();
-  if (r->cancelled) return rcwa_set_error(RCWA_E_CANCELLED, "ticket cancelled");
-  s->cv_done.wait(lk, [&] { return r->status != RCWA_E_NOTREADY; });
-  return fill(s, r, out, info);
+  ++s->waiters;
+  // cancel and destroy set the status too, so every wait ends; the entry is
+  // looked up afresh because another thread collecting the same ticket may
+  // have erased it meanwhile
+  s->cv_done.wait(lk, [&] {
+    it = s->all.find(ticket);
+    return it == s->all.end() || it->second->status != RCWA_E_NOTREADY;
+  });
+  --s->waiters;
+  rcwa_status st;
+  if (it == s->all.end()) {
+    st = rcwa_set_error(RCWA_E_ARG, "ticket collected by another call");
+  } else {
+    Request* r = it->second.get();
+    st = r->status == RCWA_E_CANCELLED ? rcwa_set_error(RCWA_E_CANCELLED, "ticket cancelled")
+                                       : fill(s, r, out, info);
+    s->all.erase(it);
+  }
+  if (s->stop && s->waiters == 0) s->cv_done.notify_all();  // destroy waits for the waiters
+  return st;
 }
 
 rcwa_status rcwa_server_cancel(rcwa_server* s, uint64_t ticket) {
   if (!s) return rcwa_set_error(RCWA_E_ARG, "null server");
-  std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->all.find(ticket);
-  if (it == s->all.end()) return rcwa_set_error(RCWA_E_ARG, "unknown ticket");
-  Request* r = it->second.get();
-  if (r->status != RCWA_E_NOTREADY || r->t_start > 0)
-    return rcwa_set_error(RCWA_E_ARG, "ticket already running or finished");
-  r->cancelled = true;  // the workers drop it from the queue
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return rcwa_set_error(RCWA_E_ARG, "unknown ticket");
+    Request* r = it->second.get();
+    if (r->status != RCWA_E_NOTREADY || r->taken)
+      return rcwa_set_error(RCWA_E_ARG, "ticket already running or finished");
+    for (auto q = s->pending.begin(); q != s->pending.end(); ++q)
+      if (*q == r) {
+        s->pending.erase(q);
+        break;
+      }
+    r->cancelled = true;
+    r->status = RCWA_E_CANCELLED;
+  }
+  s->cv_done.notify_all();
   return RCWA_OK;
 }
 
 double rcwa_server_clock(rcwa_server* s) { return s ? s->now() : 0.0; }
 
+// Requests still queued end as RCWA_E_CANCELLED; the batches already on the
+// GPU finish.  Threads blocked in rcwa_server_wait return before the server
+// is freed; no call on the server may start once destroy has been called.
 rcwa_status rcwa_server_destroy(rcwa_server* s) {
   if (!s) return RCWA_OK;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->stop = true;
-    for (Request* r : s->pending) r->cancelled = true;
+    for (Request* r : s->pending) {
+      r->cancelled = true;
+      r->status = RCWA_E_CANCELLED;
+    }
     s->pending.clear();
   }
   s->cv_work.notify_all();
   for (auto& t : s->workers) t.join();
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+  }
   for (auto& c : s->ctxs) {
     (void)hipStreamSynchronize(c->stream);
     c->release();

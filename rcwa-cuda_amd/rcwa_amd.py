@@ -52,7 +52,7 @@ class Params(ctypes.Structure):
 class Results(ctypes.Structure):
     _fields_ = [("R", ctypes.c_void_p), ("T", ctypes.c_void_p),
                 ("R_tot", ctypes.c_double), ("T_tot", ctypes.c_double),
-                ("r", ctypes.c_void_p), ("t", ctypes.c_void_p)]
+                ("r", ctypes.c_void_p), ("t", ctypes.c_void_p), ("cond", ctypes.c_double)]
 
 
 SHAPE_ELLIPSE, SHAPE_RECT, SHAPE_CIRCLE = 0, 1, 2
@@ -188,6 +188,22 @@ def _cz(a, shape=None):
     return a if shape is None else a.reshape(shape)
 
 
+def _grids(p, on_device=False):
+    """(er, ur, thick) of a workloads.Problem as the library reads them:
+    C-contiguous complex128 grids [layer][iy][ix] and float64 thicknesses,
+    with er.shape == ur.shape == (len(thick), Ny, Nx).  Checked here, at the
+    call, because the caller may have replaced p.er after constructing p."""
+    thick = np.ascontiguousarray(p.thick, dtype=np.float64).reshape(-1)
+    if on_device:
+        return None, None, thick
+    er = _cz(p.er)
+    ur = _cz(p.ur)
+    if er.ndim != 3 or er.shape != ur.shape or er.shape[0] != thick.size:
+        raise ValueError("er and ur must both have shape (len(thick), Ny, Nx); got %s, %s, %d"
+                         % (er.shape, ur.shape, thick.size))
+    return er, ur, thick
+
+
 def _fz(a):
     """Column-major complex128 array (the library's matrix layout)."""
     return np.asfortranarray(a, dtype=np.complex128)
@@ -270,14 +286,18 @@ class Context:
             ra = np.zeros((npol, 3 * P * Q), np.complex128) if fields else None
             ta = np.zeros((npol, 3 * P * Q), np.complex128) if fields else None
             outs.append((R, T, ra, ta))
+            er, ur, thick = _grids(p, on_device=device_grids is not None)
             if device_grids is not None:
                 er_p, ur_p = device_grids[i]
                 ondev = 1
+                if grid_shape is None:
+                    raise ValueError("device_grids needs grid_shape")
             else:
-                er_p, ur_p = p.er.ctypes.data, p.ur.ctypes.data
+                er_p, ur_p = er.ctypes.data, ur.ctypes.data
                 ondev = 0
-            Ny, Nx = grid_shape if grid_shape is not None else p.er.shape[1:]
-            d = Device(Nx, Ny, len(p.thick), p.Lx, p.Ly, er_p, ur_p, p.thick.ctypes.data, ondev)
+            Ny, Nx = grid_shape if grid_shape is not None else er.shape[1:]
+            d = Device(Nx, Ny, thick.size, p.Lx, p.Ly, er_p, ur_p, thick.ctypes.data, ondev)
+            keep += [er, ur, thick]
             par = Params(p.hx, p.hy, _c(p.er_ref), _c(p.ur_ref), _c(p.er_trn), _c(p.ur_trn))
             srcs = (Source * npol)(*[Source(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols])
             keep += [d, par, srcs, p]
@@ -289,12 +309,15 @@ class Context:
                 k += 1
         _check(lib().rcwa_solve_batch(self.h, cprobs, len(probs), precision, res))
         out = []
+        # per problem, per source: the conditioning bound of rcwa_results.cond
+        self.last_cond = []
         k = 0
         for i, p in enumerate(probs):
             P, Q = 2 * p.hx + 1, 2 * p.hy + 1
             npol = len(p.pols)
             R, T, ra, ta = outs[i]
             tot = np.array([[res[k + j].R_tot, res[k + j].T_tot] for j in range(npol)])
+            self.last_cond.append(np.array([res[k + j].cond for j in range(npol)]))
             k += npol
             one = (R.reshape(npol, P, Q).transpose(0, 2, 1),
                    T.reshape(npol, P, Q).transpose(0, 2, 1), tot)
@@ -507,9 +530,9 @@ class Server:
         self._meta = {}
 
     def submit(self, p):
-        L, Ny, Nx = p.er.shape
-        d = Device(Nx, Ny, L, p.Lx, p.Ly, p.er.ctypes.data, p.ur.ctypes.data,
-                   p.thick.ctypes.data, 0)
+        er, ur, thick = _grids(p)
+        L, Ny, Nx = er.shape
+        d = Device(Nx, Ny, L, p.Lx, p.Ly, er.ctypes.data, ur.ctypes.data, thick.ctypes.data, 0)
         par = Params(p.hx, p.hy, _c(p.er_ref), _c(p.ur_ref), _c(p.er_trn), _c(p.ur_trn))
         npol = len(p.pols)
         srcs = (Source * npol)(*[Source(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols])
@@ -520,6 +543,8 @@ class Server:
         return t.value
 
     def _collect(self, fn, ticket, fields=False):
+        if ticket not in self._meta:
+            raise RcwaError(1, "unknown ticket %d (never submitted, or already collected)" % ticket)
         hx, hy, npol = self._meta[ticket]
         P, Q = 2 * hx + 1, 2 * hy + 1
         R = np.zeros((npol, P * Q))
@@ -534,6 +559,8 @@ class Server:
         rc = fn(self.h, ctypes.c_uint64(ticket), res, ctypes.byref(info))
         if rc == 7:
             return None
+        # any other status is final: the server has erased the ticket
+        self._meta.pop(ticket, None)
         _check(rc)
         tot = np.array([[res[j].R_tot, res[j].T_tot] for j in range(npol)])
         info_d = {"t_submit": info.t_submit, "t_start": info.t_start,

@@ -1,0 +1,66 @@
+"""The serving form on the benchmark's own request stream: the first 16
+requests of the seeded Poisson stream (2-D lattices of random ellipses and
+rectangles on 256^2 grids, 7x7 to 25x25 harmonics -- all five sizes appear --
+and 2 to 29 layers), submitted together to one server.
+
+  - every served result equals the same request solved alone, one request
+    per call, to 1e-10;
+  - the requests with at most 11x11 harmonics match the CPU FP64 oracle per
+    order to 1e-9;
+  - the stream made lossless (every permittivity real) conserves energy,
+    R + T = 1 to 1e-10, for every request, the 25x25-harmonic 29-layer ones
+    included, where the oracle would take hours."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+import rcwa_amd as G
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PREFIX = 16
+
+
+@pytest.fixture(scope="module")
+def stream():
+    reqs = [p for _, p in W.c4_stream(count=PREFIX)]
+    assert sorted({p.hx for p in reqs}) == [3, 5, 7, 10, 12]
+    return reqs
+
+
+def _serve(reqs):
+    with G.Server(0, n_workers=2) as srv:
+        tickets = [srv.submit(p) for p in reqs]
+        return [srv.wait(t) for t in tickets]
+
+
+def test_c4_served_equals_alone_and_oracle(ctx, stream):
+    served = _serve(stream)
+    assert max(r[3]["batch_size"] for r in served) > 1
+    for i, (p, r) in enumerate(zip(stream, served)):
+        alone = ctx.solve([p])[0]
+        d = max(np.abs(r[0] - alone[0]).max(), np.abs(r[1] - alone[1]).max(),
+                np.abs(r[2] - alone[2]).max())
+        print("request %d: h=%d L=%d served vs alone %.2e" % (i, p.hx, p.L, d))
+        assert d < 1e-10
+        if p.hx <= 5:
+            Ro, To, toto = O.solve(p)
+            e = max(np.abs(r[0] - Ro).max(), np.abs(r[1] - To).max(),
+                    np.abs(r[2] - toto).max())
+            print("request %d vs oracle %.2e" % (i, e))
+            assert e < 1e-9
+
+
+def test_c4_lossless_energy(stream):
+    lossless = []
+    for p in stream:
+        q = W.Problem(p.er.real, p.ur, p.thick, p.Lx, p.Ly, p.lam0, p.theta, p.phi, p.pols,
+                      p.hx, p.hy, p.er_ref, p.ur_ref, p.er_trn, p.ur_trn)
+        lossless.append(q)
+    served = _serve(lossless)
+    for p, (R, T, tot, info) in zip(lossless, served):
+        dev = np.abs(tot.sum(axis=1) - 1.0).max()
+        print("h=%d L=%d: R + T - 1 = %.2e" % (p.hx, p.L, dev))
+        assert dev < 1e-10
+        assert R.min() > -1e-12 and T.min() > -1e-12

@@ -60,3 +60,40 @@ def test_cancel_and_poll():
         if cancelled:
             with pytest.raises(G.RcwaError):
                 srv.poll(ts[-1])
+
+
+def test_cancel_wakes_a_waiter_and_tickets_are_collected_once():
+    """A thread blocked in wait on a queued ticket returns CANCELLED when
+    another thread cancels it (it must not hang), and a collected ticket is
+    gone: a second poll reports an unknown ticket."""
+    import threading
+    probs = _mix(7, 8)
+    with G.Server(0, hbm_budget=4 << 30, n_workers=1, max_batch=1) as srv:
+        ts = [srv.submit(p) for p in probs]
+        got = {}
+
+        def waiter():
+            try:
+                got["r"] = srv.wait(ts[-1])
+            except G.RcwaError as e:
+                got["e"] = e.code
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        time.sleep(0.05)
+        try:
+            srv.cancel(ts[-1])
+            cancelled = True
+        except G.RcwaError:
+            cancelled = False                    # already taken into a batch
+        th.join(timeout=60)
+        assert not th.is_alive(), "wait did not return"
+        if cancelled:
+            assert got.get("e") == 8             # RCWA_E_CANCELLED
+        else:
+            assert "r" in got
+        assert srv.wait(ts[0]) is not None
+        with pytest.raises(G.RcwaError):
+            srv.poll(ts[0])                      # already collected
+        for t in ts[1:-1]:
+            srv.wait(t)
