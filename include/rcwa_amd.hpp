@@ -21,6 +21,7 @@
 #include <ostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rcwa_amd.h"
@@ -72,8 +73,67 @@ class Dense {
     for (const T& v : a_) s += v;
     return s;
   }
+  // Eigen's LinSpaced(size, low, high) for vectors
+  static Dense LinSpaced(long n, T lo, T hi) {
+    Dense v(n);
+    for (long i = 0; i < n; ++i) v(i) = n > 1 ? lo + (hi - lo) * T((double)i / (n - 1)) : hi;
+    return v;
+  }
+  Dense transpose() const {
+    Dense t(c_, r_);
+    for (long j = 0; j < c_; ++j)
+      for (long i = 0; i < r_; ++i) t(j, i) = (*this)(i, j);
+    return t;
+  }
+  Dense conjugate() const {
+    Dense t(*this);
+    for (T& v : t.a_) v = conj_(v);
+    return t;
+  }
+  Dense adjoint() const { return transpose().conjugate(); }
+  // copies of a block, a column and a row (reads; write blocks with set_block)
+  Dense block(long i, long j, long r, long c) const {
+    Dense b(r, c);
+    for (long q = 0; q < c; ++q)
+      for (long p = 0; p < r; ++p) b(p, q) = (*this)(i + p, j + q);
+    return b;
+  }
+  void set_block(long i, long j, const Dense& b) {
+    for (long q = 0; q < b.cols(); ++q)
+      for (long p = 0; p < b.rows(); ++p) (*this)(i + p, j + q) = b(p, q);
+  }
+  Dense col(long j) const { return block(0, j, r_, 1); }
+  Dense row(long i) const { return block(i, 0, 1, c_); }
+  // Eigen's replicate(rows_factor, cols_factor)
+  Dense replicate(long fr, long fc) const {
+    Dense m(r_ * fr, c_ * fc);
+    for (long bj = 0; bj < fc; ++bj)
+      for (long bi = 0; bi < fr; ++bi) m.set_block(bi * r_, bj * c_, *this);
+    return m;
+  }
+  double norm() const {
+    double s = 0;
+    for (const T& v : a_) s += std::norm(v);
+    return std::sqrt(s);
+  }
+  Dense& operator+=(const Dense& b) {
+    for (size_t i = 0; i < a_.size(); ++i) a_[i] += b.a_[i];
+    return *this;
+  }
+  Dense& operator-=(const Dense& b) {
+    for (size_t i = 0; i < a_.size(); ++i) a_[i] -= b.a_[i];
+    return *this;
+  }
+  Dense& operator*=(T s) {
+    for (T& v : a_) v *= s;
+    return *this;
+  }
 
  private:
+  static T conj_(const T& v) {
+    if constexpr (std::is_same<T, Complex>::value) return std::conj(v);
+    else return v;
+  }
   long r_ = 0, c_ = 0;
   std::vector<T> a_;
 };
@@ -101,6 +161,88 @@ inline void set_context(rcwa_ctx* c) { default_ctx() = c; }
 inline const rcwa_c128* C(const Complex* p) { return reinterpret_cast<const rcwa_c128*>(p); }
 inline rcwa_c128* C(Complex* p) { return reinterpret_cast<rcwa_c128*>(p); }
 
+template <typename T>
+Dense<T> operator+(Dense<T> a, const Dense<T>& b) { return a += b; }
+template <typename T>
+Dense<T> operator-(Dense<T> a, const Dense<T>& b) { return a -= b; }
+template <typename T>
+Dense<T> operator-(Dense<T> a) { return a *= T(-1); }
+template <typename T>
+Dense<T> operator*(Dense<T> a, T s) { return a *= s; }
+template <typename T>
+Dense<T> operator*(T s, Dense<T> a) { return a *= s; }
+
+// Matrix product (Eigen's operator*): complex products from 64 x 64 x 64
+// upwards run on the GPU (rcwa_zgemm), smaller ones on the host.
+template <typename T>
+Dense<T> operator*(const Dense<T>& a, const Dense<T>& b) {
+  if (a.cols() != b.rows()) throw Error(RCWA_E_ARG, "operator*: inner dimensions differ");
+  Dense<T> c(a.rows(), b.cols());
+  if constexpr (std::is_same<T, Complex>::value) {
+    if (a.rows() * b.cols() * a.cols() >= 64L * 64 * 64) {
+      rcwa_ctx* x = nullptr;
+      if (!default_ctx()) check(rcwa_ctx_create(0, 0, &default_ctx()));
+      x = default_ctx();
+      check(rcwa_zgemm(x, (int)a.rows(), (int)b.cols(), (int)a.cols(), 1, C(a.data()),
+                       C(b.data()), C(c.data())));
+      return c;
+    }
+  }
+  for (long j = 0; j < b.cols(); ++j)
+    for (long k = 0; k < a.cols(); ++k) {
+      const T bk = b(k, j);
+      for (long i = 0; i < a.rows(); ++i) c(i, j) += a(i, k) * bk;
+    }
+  return c;
+}
+
+// Eigen's PartialPivLU (rcwa.cpp takes .lu() of every matrix it solves
+// with): row pivoting on the largest magnitude per column, on the host.
+template <typename T>
+class PartialPivLU {
+ public:
+  explicit PartialPivLU(const Dense<T>& a) : f_(a), piv_(a.rows()) {
+    if (a.rows() != a.cols()) throw Error(RCWA_E_ARG, "lu: matrix is not square");
+    const long n = a.rows();
+    for (long k = 0; k < n; ++k) {
+      long p = k;
+      for (long i = k + 1; i < n; ++i)
+        if (std::abs(f_(i, k)) > std::abs(f_(p, k))) p = i;
+      piv_[k] = p;
+      if (p != k)
+        for (long j = 0; j < n; ++j) std::swap(f_(k, j), f_(p, j));
+      if (f_(k, k) == T(0)) continue;
+      for (long i = k + 1; i < n; ++i) f_(i, k) /= f_(k, k);
+      for (long j = k + 1; j < n; ++j)
+        for (long i = k + 1; i < n; ++i) f_(i, j) -= f_(i, k) * f_(k, j);
+    }
+  }
+  Dense<T> solve(const Dense<T>& b) const {
+    const long n = f_.rows();
+    if (b.rows() != n) throw Error(RCWA_E_ARG, "lu solve: shape");
+    Dense<T> x(b);
+    for (long j = 0; j < x.cols(); ++j) {
+      for (long k = 0; k < n; ++k)
+        if (piv_[k] != k) std::swap(x(k, j), x(piv_[k], j));
+      for (long k = 0; k < n; ++k)
+        for (long i = k + 1; i < n; ++i) x(i, j) -= f_(i, k) * x(k, j);
+      for (long k = n - 1; k >= 0; --k) {
+        x(k, j) /= f_(k, k);
+        for (long i = 0; i < k; ++i) x(i, j) -= f_(i, k) * x(k, j);
+      }
+    }
+    return x;
+  }
+  Dense<T> inverse() const { return solve(Dense<T>::Identity(f_.rows(), f_.rows())); }
+
+ private:
+  Dense<T> f_;
+  std::vector<long> piv_;
+};
+
+template <typename T>
+PartialPivLU<T> lu(const Dense<T>& a) { return PartialPivLU<T>(a); }
+
 }  // namespace rcwa_amd
 
 using Matrix = rcwa_amd::Dense<Complex>;
@@ -108,10 +250,105 @@ using Vector = rcwa_amd::Dense<Complex>;
 using Real_Matrix = rcwa_amd::Dense<Real>;
 using Real_Vector = rcwa_amd::Dense<Real>;
 
+// rcwa.hpp:17: 3-vector of complex numbers (brace-initialised to zeros).
+struct Vec_3d {
+  Complex v[3]{};
+  Complex& operator()(int i) { return v[i]; }
+  const Complex& operator()(int i) const { return v[i]; }
+  Complex& operator[](int i) { return v[i]; }
+  const Complex& operator[](int i) const { return v[i]; }
+  static constexpr long size() { return 3; }
+  Complex* data() { return v; }
+  const Complex* data() const { return v; }
+  double norm() const { return std::sqrt(std::norm(v[0]) + std::norm(v[1]) + std::norm(v[2])); }
+  Vec_3d normalized() const {
+    const double n = norm();
+    return Vec_3d{{v[0] / n, v[1] / n, v[2] / n}};
+  }
+  // Eigen's cross(): this x b
+  Vec_3d cross(const Vec_3d& b) const {
+    return Vec_3d{{v[1] * b.v[2] - v[2] * b.v[1], v[2] * b.v[0] - v[0] * b.v[2],
+                   v[0] * b.v[1] - v[1] * b.v[0]}};
+  }
+};
+inline Vec_3d operator+(const Vec_3d& a, const Vec_3d& b) {
+  return Vec_3d{{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]}};
+}
+inline Vec_3d operator*(Real s, const Vec_3d& a) { return Vec_3d{{s * a.v[0], s * a.v[1], s * a.v[2]}}; }
+inline Vec_3d operator*(Complex s, const Vec_3d& a) {
+  return Vec_3d{{s * a.v[0], s * a.v[1], s * a.v[2]}};
+}
+
 // rcwa.hpp:24-28
 inline Complex unsigned_sqrt(Complex z) {
   if (z.imag() == 0.0) z = Complex(z.real(), 0.0);
   return std::sqrt(z);
+}
+
+// rcwa.hpp:37-44: the elementwise form, for vectors and matrices.
+inline Matrix unsigned_sqrt(const Matrix& m) {
+  Matrix r(m.rows(), m.cols());
+  for (long j = 0; j < m.cols(); ++j)
+    for (long i = 0; i < m.rows(); ++i) r(i, j) = unsigned_sqrt(m(i, j));
+  return r;
+}
+
+// rcwa.hpp:218-224: X = x^T repeated over y.size() rows, Y = y repeated over
+// x.size() columns.
+template <typename Scalar>
+void MeshGrid(const rcwa_amd::Dense<Scalar>& x, const rcwa_amd::Dense<Scalar>& y,
+              rcwa_amd::Dense<Scalar>& X, rcwa_amd::Dense<Scalar>& Y) {
+  X = x.transpose().replicate(y.size(), 1);
+  Y = y.replicate(1, x.size());
+}
+
+// rcwa.hpp:227-230
+struct EigenvalSolverResults {
+  Vector eigenvalues{};
+  Matrix eigenvectors{};
+};
+
+namespace rcwa_amd {
+// The eigensolver the reference takes from Eigen (ComplexEigenSolver,
+// rcwa.cpp:298-306), on the GPU: eigenvalues and unit 2-norm eigenvectors of
+// a general complex matrix.  Their order and phases differ from Eigen's
+// (which sorts them); the S-matrices built from them do not depend on either.
+template <typename MatrixType>
+class ComplexEigenSolver {
+ public:
+  ComplexEigenSolver() = default;
+  explicit ComplexEigenSolver(const MatrixType& a, bool computeEigenvectors = true) {
+    compute(a, computeEigenvectors);
+  }
+  ComplexEigenSolver& compute(const MatrixType& a, bool computeEigenvectors = true) {
+    (void)computeEigenvectors;
+    if (a.rows() != a.cols()) throw Error(RCWA_E_ARG, "ComplexEigenSolver: not square");
+    const long n = a.rows();
+    w_ = Vector(n);
+    v_ = Matrix(n, n);
+    const rcwa_status st =
+        rcwa_eig(ctx(), (int)n, 1, C(a.data()), C(w_.data()), C(v_.data()));
+    ok_ = st == RCWA_OK;
+    if (st != RCWA_OK && st != RCWA_E_EIG_NOCONV) check(st);
+    return *this;
+  }
+  const Vector& eigenvalues() const { return w_; }
+  const Matrix& eigenvectors() const { return v_; }
+  bool converged() const { return ok_; }
+
+ private:
+  Vector w_;
+  Matrix v_;
+  bool ok_ = false;
+};
+}  // namespace rcwa_amd
+
+// rcwa.hpp:233 / rcwa.cpp:617-621.  The reference aborts when the solver did
+// not converge (rcwa.cpp:300); this throws rcwa_amd::Error(RCWA_E_EIG_NOCONV).
+inline EigenvalSolverResults extraction(rcwa_amd::ComplexEigenSolver<Matrix>&& solver) {
+  if (!solver.converged())
+    throw rcwa_amd::Error(RCWA_E_EIG_NOCONV, "eigensolver QR did not converge");
+  return EigenvalSolverResults{solver.eigenvalues(), solver.eigenvectors()};
 }
 
 // rcwa.hpp:48-78

@@ -1,6 +1,11 @@
 """The rcwa.hpp drop-in (include/rcwa_amd.hpp): a caller written like
 rcwa3d/src/main.cpp:112-198 runs the stage functions one by one on the GPU
-and reproduces the oracle, as does the one-call Solve()."""
+and reproduces the oracle, as does the one-call Solve().  The same caller
+then uses every helper of rcwa.hpp -- Vec_3d (rcwa.hpp:17, the polarisation
+of rcwa.cpp:481-498), unsigned_sqrt over a matrix (rcwa.hpp:37-44),
+MeshGrid (rcwa.hpp:218-224), ComplexEigenSolver + extraction (rcwa.hpp:227-233,
+rcwa.cpp:617-621), PartialPivLU and the matrix product -- each checked
+against what it must equal."""
 import os
 import subprocess
 
@@ -40,7 +45,8 @@ def test_dropin_main_matches_oracle(seed, hx, hy, L):
         out = subprocess.run([EXE], input=_stdin(p, pol), capture_output=True, text=True,
                              timeout=120)
         assert out.returncode == 0, out.stderr
-        vals = [list(map(float, l.split())) for l in out.stdout.strip().splitlines()]
+        vals = [list(map(float, l.split())) for l in out.stdout.strip().splitlines()
+                if not l.startswith("helper ")]
         PQ = int(vals[0][0])
         staged = np.array(vals[1:1 + PQ])
         tot_s = vals[1 + PQ]
@@ -49,6 +55,11 @@ def test_dropin_main_matches_oracle(seed, hx, hy, L):
         R, T, tot = O.solve(p)
         Rf = R[pol].reshape(-1, order="F")
         Tf = T[pol].reshape(-1, order="F")
+        helpers = {l.split()[1]: float(l.split()[2])
+                   for l in out.stdout.strip().splitlines() if l.startswith("helper ")}
+        assert set(helpers) == {"vec3d", "usqrt", "meshgrid", "eig", "lu", "gemm"}, helpers
+        assert helpers["vec3d"] < 1e-14 and helpers["usqrt"] == 0 and helpers["meshgrid"] == 0
+        assert helpers["eig"] < 1e-12 and helpers["lu"] < 1e-13 and helpers["gemm"] < 1e-14
         for arr, tt in ((staged, tot_s), (one, tot_o)):
             assert np.abs(arr[:, 0] - Rf).max() < 1e-9
             assert np.abs(arr[:, 1] - Tf).max() < 1e-9
