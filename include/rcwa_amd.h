@@ -230,6 +230,27 @@ rcwa_status rcwa_convmat_batched(rcwa_ctx* ctx, const rcwa_c128* grids_dev, int 
                                  int Nx_harmonics, int Ny_harmonics, int batch,
                                  rcwa_c128* C_dev);
 
+/* rcwa.hpp:184 SMatrixLayer for `batch` layers, HBM-resident: erc_dev and
+ * urc_dev are device pointers to batch x (PQ x PQ) convolution matrices,
+ * S11_dev and S12_dev device pointers to batch x (2PQ x 2PQ) outputs
+ * (S22 = S11, S21 = S12, rcwa.cpp:351).  Each layer has its own incidence:
+ * Kx, Ky are host arrays of batch x PQ (from rcwa_wave_vectors), k0t a host
+ * array of k0 * t per layer.  Work buffers come from the context's arena.
+ * Enqueued on the context stream; returns when S11 and S12 are complete. */
+rcwa_status rcwa_layer_smatrix_batched(rcwa_ctx* ctx, int PQ, int batch, const rcwa_c128* erc_dev,
+                                       const rcwa_c128* urc_dev, const rcwa_c128* Kx,
+                                       const rcwa_c128* Ky, const double* k0t,
+                                       rcwa_c128* S11_dev, rcwa_c128* S12_dev);
+
+/* rcwa.hpp:214 RedhefferProduct on `batch` pairs, HBM-resident: A_dev[4],
+ * B_dev[4] and S_dev[4] (host arrays of four device pointers, blocks S11,
+ * S12, S21, S22) each point to batch x (n x n) column-major blocks.  S may
+ * alias A (in-place update of A), not B.  Work buffers come from the
+ * context's arena.  Returns when S is complete. */
+rcwa_status rcwa_star_product_batched(rcwa_ctx* ctx, int n, int batch,
+                                      const rcwa_c128* const* A_dev,
+                                      const rcwa_c128* const* B_dev, rcwa_c128* const* S_dev);
+
 /* ----- structure rasterisation on the device ----- */
 
 /* The reference rasterises its structure on the host before ConvMat

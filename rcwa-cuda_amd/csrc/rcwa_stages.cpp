@@ -62,6 +62,31 @@ rcwa_status check_info(Ctx& c, int* dinfo, int count) {
 
 }  // namespace
 
+// S = A * B for device blocks (the work of rcwa_star_product_batched and its
+// allocation-free form): A is copied into S, the product forms S in place.
+// t[5] are batch x n x n work slots, gj the Gauss-Jordan workspace, info
+// batch ints already cleared (set where an inverse met an exactly zero
+// pivot).  Enqueued on the context stream; the caller reads info.
+rcwa_status star_product_device(Ctx& c, int n, int batch, const rcwa_c128* const* A_dev,
+                                const rcwa_c128* const* B_dev, rcwa_c128* const* S_dev, zc* t[5],
+                                void* gj, int* dinfo) {
+  const long long n2 = (long long)n * n;
+  for (int k = 0; k < 4; ++k)
+    for (int q = 0; q < 4; ++q)
+      if ((const void*)S_dev[k] == (const void*)B_dev[q])
+        return rcwa_set_error(RCWA_E_ARG, "S must not alias B");
+  zc* a[4];
+  ZCMat Bm[4];
+  for (int k = 0; k < 4; ++k) {
+    a[k] = (zc*)S_dev[k];
+    Bm[k] = ZCMat{(const zc*)B_dev[k], n, n2};
+    if ((const void*)A_dev[k] != (const void*)S_dev[k])
+      CK(zcopy_batched(c.stream, n, n, ZCMat{(const zc*)A_dev[k], n, n2}, ZMat{a[k], n, n2},
+                       batch));
+  }
+  return star_product(c, n, batch, a, Bm, t, gj, dinfo, true);
+}
+
 extern "C" {
 
 rcwa_status rcwa_convmat(rcwa_ctx* ctx, const rcwa_c128* field, int layer, int Nx, int Ny,
@@ -293,6 +318,91 @@ rcwa_status rcwa_star_product(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* 
   for (int k = 0; k < 4; ++k)
     if ((st = d2h(c, S[k], a[k], blk))) return st;
   return RCWA_OK;
+}
+
+rcwa_status rcwa_layer_smatrix_batched(rcwa_ctx* ctx, int PQ, int batch, const rcwa_c128* erc_dev,
+                                       const rcwa_c128* urc_dev, const rcwa_c128* Kx,
+                                       const rcwa_c128* Ky, const double* k0t,
+                                       rcwa_c128* S11_dev, rcwa_c128* S12_dev) {
+  if (!ctx || PQ <= 0 || batch <= 0 || !erc_dev || !urc_dev || !Kx || !Ky || !k0t || !S11_dev ||
+      !S12_dev)
+    return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  Ctx& c = *C(ctx);
+  CKH(hipSetDevice(c.device));
+  c.precision = RCWA_FP64;
+  const int N = PQ, n = 2 * N;
+  const size_t N2 = (size_t)N * N;
+  for (int b = 0; b < batch; ++b)
+    if (!std::isfinite(k0t[b]) || k0t[b] < 0)
+      return rcwa_set_error(RCWA_E_ARG, "k0 t must be finite and >= 0");
+  const size_t bytes = (batch * (6 * (size_t)N + 4 * (size_t)n)) * sizeof(zc) +
+                       batch * (sizeof(double) + 2 * sizeof(int)) + layer_work_bytes(N, batch) +
+                       64 * 256;
+  rcwa_status st = c.reserve(bytes);
+  if (st) return st;
+  zc* dKx = (zc*)c.take((size_t)batch * N * sizeof(zc));
+  zc* dKy = (zc*)c.take((size_t)batch * N * sizeof(zc));
+  zc* dv0 = (zc*)c.take((size_t)batch * 4 * N * sizeof(zc));
+  zc* lam2 = (zc*)c.take((size_t)batch * n * sizeof(zc));
+  zc* lam = (zc*)c.take((size_t)batch * n * sizeof(zc));
+  zc* ilam = (zc*)c.take((size_t)batch * n * sizeof(zc));
+  zc* X = (zc*)c.take((size_t)batch * n * sizeof(zc));
+  double* dk = (double*)c.take(batch * sizeof(double));
+  int* dinfo = (int*)c.take(2 * batch * sizeof(int));
+  LayerWork w;
+  w.carve(c, N, batch);
+  if ((st = c.arena_check()) != RCWA_OK) return st;
+  std::vector<cplx> hv0((size_t)batch * 4 * N);
+  for (int b = 0; b < batch; ++b) {
+    std::vector<cplx> v0 =
+        rcwa_host::gap_v0_diagonals(vec(Kx + (size_t)b * N, N), vec(Ky + (size_t)b * N, N));
+    std::copy(v0.begin(), v0.end(), hv0.begin() + (size_t)b * 4 * N);
+  }
+  for (const cplx& z : hv0)
+    if (!std::isfinite(z.real()) || !std::isfinite(z.imag()))
+      return rcwa_set_error(RCWA_E_SINGULAR, "k_z = 0 in the gap medium (Wood anomaly)");
+  if ((st = h2d(c, dKx, Kx, (size_t)batch * N * sizeof(zc)))) return st;
+  if ((st = h2d(c, dKy, Ky, (size_t)batch * N * sizeof(zc)))) return st;
+  if ((st = h2d(c, dv0, hv0.data(), hv0.size() * sizeof(zc)))) return st;
+  if ((st = h2d(c, dk, k0t, batch * sizeof(double)))) return st;
+  CKH(hipMemsetAsync(dinfo, 0, 2 * batch * sizeof(int), c.stream));
+  const long long n2 = (long long)n * n;
+  st = layer_stage(c, N, batch, ZCMat{(const zc*)erc_dev, N, (long long)N2},
+                   ZCMat{(const zc*)urc_dev, N, (long long)N2}, dKx, dKy, dv0, dk, lam2, lam, ilam,
+                   X, ZMat{(zc*)S11_dev, n, n2}, ZMat{(zc*)S12_dev, n, n2}, w, dinfo,
+                   dinfo + batch);
+  if (st) return st;
+  std::vector<int> hinfo(2 * batch);
+  if ((st = d2h(c, hinfo.data(), dinfo, 2 * batch * sizeof(int)))) return st;
+  for (int b = 0; b < batch; ++b) {
+    if (hinfo[b]) return rcwa_set_error(RCWA_E_EIG_NOCONV, "eigensolver QR did not converge");
+    if (hinfo[batch + b]) return rcwa_set_error(RCWA_E_SINGULAR, "exactly singular matrix inverted");
+  }
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_star_product_batched(rcwa_ctx* ctx, int n, int batch,
+                                      const rcwa_c128* const* A_dev,
+                                      const rcwa_c128* const* B_dev, rcwa_c128* const* S_dev) {
+  if (!ctx || n <= 0 || batch <= 0 || !A_dev || !B_dev || !S_dev)
+    return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  for (int k = 0; k < 4; ++k)
+    if (!A_dev[k] || !B_dev[k] || !S_dev[k]) return rcwa_set_error(RCWA_E_ARG, "null block");
+  Ctx& c = *C(ctx);
+  CKH(hipSetDevice(c.device));
+  c.precision = RCWA_FP64;
+  const size_t n2 = (size_t)n * n, blk = batch * n2 * sizeof(zc);
+  rcwa_status st = c.reserve(5 * blk + gj_workspace_bytes(n, batch) + batch * sizeof(int) +
+                             64 * 256);
+  if (st) return st;
+  zc* t[5];
+  for (auto& p : t) p = (zc*)c.take(blk);
+  void* gj = c.take(gj_workspace_bytes(n, batch));
+  int* dinfo = (int*)c.take(batch * sizeof(int));
+  if ((st = c.arena_check()) != RCWA_OK) return st;
+  CKH(hipMemsetAsync(dinfo, 0, batch * sizeof(int), c.stream));
+  if ((st = star_product_device(c, n, batch, A_dev, B_dev, S_dev, t, gj, dinfo))) return st;
+  return check_info(c, dinfo, batch);
 }
 
 rcwa_status rcwa_eig(rcwa_ctx* ctx, int n, int batch, const rcwa_c128* A, rcwa_c128* w,

@@ -84,6 +84,12 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
                          void* gjws, int* info_sing, bool keep_slots);
 
+// S = A * B on device blocks (rcwa_star_product_batched and its
+// allocation-free form); t[5] work slots, gj workspace, info cleared.
+rcwa_status star_product_device(Ctx& c, int n, int batch, const rcwa_c128* const* A_dev,
+                                const rcwa_c128* const* B_dev, rcwa_c128* const* S_dev, zc* t[5],
+                                void* gj, int* dinfo);
+
 // A problem with the place its results go.
 struct ProblemRef {
   rcwa_problem p;

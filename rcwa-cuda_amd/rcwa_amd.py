@@ -64,6 +64,43 @@ class Shape(ctypes.Structure):
                 ("value", c128)]
 
 
+class DeviceBuffer:
+    """Device memory for tests and callers of the HBM-resident entry points:
+    .ptr, upload(array), download(shape)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = _hip_malloc(max(self.nbytes, 16))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        rc = _hip().hipMemcpy(ctypes.c_void_p(self.ptr), ctypes.c_void_p(a.ctypes.data),
+                              ctypes.c_size_t(a.nbytes), 1)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+
+    def download(self, shape, dtype=np.complex128):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        rc = _hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.ptr),
+                              ctypes.c_size_t(out.nbytes), 2)
+        if rc != 0:
+            raise RuntimeError("hipMemcpy failed: %d" % rc)
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip().hipFree(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class DeviceGrid:
     """Device memory of rcwa_rasterize's output: num_layers x Ny x Nx
     complex128 ([layer][iy][ix]); .ptr for device_grids in Context.solve."""
@@ -150,7 +187,8 @@ EXPORTS = [
     "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
     "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
-    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_rasterize", "rcwa_sweep_create",
+    "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_layer_smatrix_batched",
+    "rcwa_star_product_batched", "rcwa_rasterize", "rcwa_sweep_create",
     "rcwa_sweep_run", "rcwa_sweep_destroy", "rcwa_server_create",
     "rcwa_server_submit",
     "rcwa_server_poll", "rcwa_server_wait", "rcwa_server_cancel", "rcwa_server_clock",
@@ -387,6 +425,24 @@ class Context:
                                        arr(*[x.ctypes.data for x in Bs]),
                                        arr(*[x.ctypes.data for x in Ss])))
         return tuple(_unstack_f(x, b, n, n) for x in Ss)
+
+    def layer_smatrix_batched(self, erc_dev, urc_dev, PQ, batch, Kx, Ky, k0t, S11_dev, S12_dev):
+        """rcwa_layer_smatrix_batched on device buffers (addresses): per-layer
+        Kx, Ky [batch][PQ] and k0t [batch] on the host."""
+        Kx = _cz(Kx).reshape(batch, PQ)
+        Ky = _cz(Ky).reshape(batch, PQ)
+        k0t = np.ascontiguousarray(k0t, np.float64).reshape(batch)
+        _check(lib().rcwa_layer_smatrix_batched(
+            self.h, PQ, batch, ctypes.c_void_p(erc_dev), ctypes.c_void_p(urc_dev), _ptr(Kx),
+            _ptr(Ky), _ptr(k0t), ctypes.c_void_p(S11_dev), ctypes.c_void_p(S12_dev)))
+
+    def star_batched(self, n, batch, A_dev, B_dev, S_dev, noalloc=False):
+        """rcwa_star_product_batched (or its allocation-free form) on device
+        blocks: A_dev, B_dev, S_dev are four device addresses each."""
+        arr = ctypes.c_void_p * 4
+        fn = (lib().rcwa_star_product_batched_noalloc if noalloc
+              else lib().rcwa_star_product_batched)
+        _check(fn(self.h, n, batch, arr(*A_dev), arr(*B_dev), arr(*S_dev)))
 
     def eig(self, mats):
         b, n = len(mats), mats[0].shape[0]
