@@ -251,6 +251,20 @@ rcwa_status rcwa_star_product_batched(rcwa_ctx* ctx, int n, int batch,
                                       const rcwa_c128* const* A_dev,
                                       const rcwa_c128* const* B_dev, rcwa_c128* const* S_dev);
 
+/* Allocation-free form of rcwa_star_product_batched: same arguments, same
+ * kernels and results, but no device memory is allocated -- its work slots,
+ * Gauss-Jordan workspace and flags live in a fixed device array of the
+ * library (52 MiB per GPU, held until the process ends).  No counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  Limits:
+ * n <= 1024, batch <= 64, n * n * batch <= 524288; larger calls return
+ * RCWA_E_ARG.  All calls on one GPU share the array, so calls that overlap on
+ * one GPU race: issue them one at a time per device. */
+rcwa_status rcwa_star_product_batched_noalloc(rcwa_ctx* ctx, int n, int batch,
+                                              const rcwa_c128* const* A_dev,
+                                              const rcwa_c128* const* B_dev,
+                                              rcwa_c128* const* S_dev);
+
 /* ----- structure rasterisation on the device ----- */
 
 /* The reference rasterises its structure on the host before ConvMat

@@ -188,7 +188,7 @@ EXPORTS = [
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
     "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
     "rcwa_efficiencies", "rcwa_convmat_batched", "rcwa_layer_smatrix_batched",
-    "rcwa_star_product_batched", "rcwa_rasterize", "rcwa_sweep_create",
+    "rcwa_star_product_batched", "rcwa_star_product_batched_noalloc", "rcwa_rasterize", "rcwa_sweep_create",
     "rcwa_sweep_run", "rcwa_sweep_destroy", "rcwa_server_create",
     "rcwa_server_submit",
     "rcwa_server_poll", "rcwa_server_wait", "rcwa_server_cancel", "rcwa_server_clock",
