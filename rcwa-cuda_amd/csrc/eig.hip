@@ -1378,6 +1378,807 @@ __global__ __launch_bounds__(NT) void schur_pipe_kernel(int n, ZMat Tm, ZMat Zm,
   }
 }
 
+
+constexpr int GAP_MS = 2;  // positions between the chains of a multishift pass
+
+// ---------------------------------------------------------------------------
+// Multishift QR with aggressive early deflation (the sweeps of
+// schur_ms_kernel).  Each pass of the iteration first looks at the trailing
+// NW x NW window of the active block: its complex Schur form T_w = V^H H_w V
+// (single-shift QR inside LDS), the spike s V(0, :)^H that the transform
+// leaves in the column left of the window (s = H(kwtop, kwtop-1)), and the
+// eigenvalues whose spike entry is negligible deflate at once; the others
+// are moved to the top of the window (swaps of adjacent diagonal entries), the
+// window is reduced back to Hessenberg form, and the bottom NC of them become
+// the shifts of the pass: NC single-shift sweeps chained two positions apart.
+// This is LAPACK's zlaqr0/zlaqr3 scheme (Braman, Byers and Mathias, 2002)
+// with single-shift bulges.  V reaches the Schur vectors as a dense block of
+// the rotation log (qr_apply_ms_kernel) and the rows of the active block above
+// the window as a product on the host of the workgroup.
+
+// Dense NW x NW unitary blocks of the AED steps, per matrix: block q acts on
+// rows/columns k0[q] .. k0[q] + nw[q] - 1 after the first pos[q] logged
+// sweeps, V[q] with leading dimension NW.
+struct BlockLog {
+  zc* V;
+  int* pos;
+  int* k0;
+  int* nw;
+  int* nblk;  // blocks logged per matrix in this launch
+  int bcap;   // blocks per matrix
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Complex Schur form of the nw x nw upper Hessenberg Tw (leading dimension
+// NW, nw <= 64) by single-shift QR sweeps, each rotation applied to the whole
+// window and accumulated into Vw (starts as the identity).  One wave; every
+// lane follows the same control flow from LDS values.  Wilkinson shifts with
+// the exceptional shifts of schur_pipe_kernel at the 10th and 20th sweep on
+// one block.  Returns false when the sweep budget runs out; *sweeps counts
+// the sweeps done.
+template <int NW>
+__device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, int& sweeps) {
+  int hi = nw - 1, it = 0;
+  const double eps = DBL_EPSILON;
+  while (hi > 0) {
+    const int k = hi - lane;
+    bool neg = false;
+    if (k >= 1) {
+      const double sub = zabs1(Tw[k + (k - 1) * NW]);
+      const double tst = zabs1(Tw[(k - 1) + (k - 1) * NW]) + zabs1(Tw[k + k * NW]);
+      neg = sub <= eps * tst || sub < DBL_MIN;
+    }
+    const unsigned long long msk = __ballot(neg);
+    const int lo = msk ? hi - (__ffsll((long long)msk) - 1) : 0;
+    wave_sync();
+    if (lo == hi) {
+      if (lane == 0) Tw[hi + (hi - 1) * NW] = zmake(0, 0);
+      wave_sync();
+      --hi;
+      it = 0;
+      continue;
+    }
+    if (sweeps >= budget) return false;
+    ++sweeps;
+    ++it;
+    if (lo > 0 && lane == 0) Tw[lo + (lo - 1) * NW] = zmake(0, 0);
+    wave_sync();
+    zc mu;
+    if (it == 10)
+      mu = zadd(Tw[lo + lo * NW], zmake(0.75 * fabs(Tw[(lo + 1) + lo * NW].x), 0));
+    else if (it == 20)
+      mu = zadd(Tw[hi + hi * NW], zmake(0.75 * fabs(Tw[hi + (hi - 1) * NW].x), 0));
+    else
+      mu = wilkinson(Tw[(hi - 1) + (hi - 1) * NW], Tw[(hi - 1) + hi * NW], Tw[hi + (hi - 1) * NW],
+                     Tw[hi + hi * NW]);
+    for (int q = lo; q < hi; ++q) {
+      zc f, g;
+      if (q == lo) {
+        f = zsub(Tw[lo + lo * NW], mu);
+        g = Tw[(lo + 1) + lo * NW];
+      } else {
+        f = Tw[q + (q - 1) * NW];
+        g = Tw[(q + 1) + (q - 1) * NW];
+      }
+      double c;
+      zc s, r;
+      givens(f, g, c, s, r);
+      wave_sync();
+      if (q > lo && lane == 0) {
+        Tw[q + (q - 1) * NW] = r;
+        Tw[(q + 1) + (q - 1) * NW] = zmake(0, 0);
+      }
+      const zc sc = zconj(s);
+      const int j = q + lane;  // row rotation: columns q .. nw-1
+      if (j < nw) {
+        zc* p0 = &Tw[q + j * NW];
+        const zc x = p0[0], y = p0[1];
+        p0[0] = zadd(zscale(x, c), zmul(s, y));
+        p0[1] = zsub(zscale(y, c), zmul(sc, x));
+      }
+      wave_sync();
+      const int i = lane;  // column rotation: rows 0 .. min(q + 2, hi), and Vw
+      if (i <= min(q + 2, hi)) {
+        zc* q0 = &Tw[i + q * NW];
+        const zc x = q0[0], y = q0[NW];
+        q0[0] = zadd(zscale(x, c), zmul(sc, y));
+        q0[NW] = zsub(zscale(y, c), zmul(s, x));
+      }
+      if (i < nw) {
+        zc* v0 = &Vw[i + q * NW];
+        const zc x = v0[0], y = v0[NW];
+        v0[0] = zadd(zscale(x, c), zmul(sc, y));
+        v0[NW] = zsub(zscale(y, c), zmul(s, x));
+      }
+      wave_sync();
+    }
+  }
+  return true;
+}
+
+// Swaps the adjacent diagonal entries q, q+1 of the upper triangular Tw
+// (LAPACK's ztrexc step): a rotation from (T(q, q+1), T(q+1, q+1) - T(q, q))
+// on rows q, q+1 right of the pair, on columns q, q+1 above it and on Vw; the
+// diagonal entries trade places and T(q, q+1) keeps its value.
+template <int NW>
+__device__ void swap_adjacent(zc* Tw, zc* Vw, int nw, int q, int lane) {
+  const zc t11 = Tw[q + q * NW], t22 = Tw[(q + 1) + (q + 1) * NW];
+  double c;
+  zc s, r;
+  givens(Tw[q + (q + 1) * NW], zsub(t22, t11), c, s, r);
+  wave_sync();
+  const zc sc = zconj(s);
+  const int j = q + 2 + lane;
+  if (j < nw) {
+    zc* p0 = &Tw[q + j * NW];
+    const zc x = p0[0], y = p0[1];
+    p0[0] = zadd(zscale(x, c), zmul(s, y));
+    p0[1] = zsub(zscale(y, c), zmul(sc, x));
+  }
+  const int i = lane;
+  if (i < q) {
+    zc* q0 = &Tw[i + q * NW];
+    const zc x = q0[0], y = q0[NW];
+    q0[0] = zadd(zscale(x, c), zmul(sc, y));
+    q0[NW] = zsub(zscale(y, c), zmul(s, x));
+  }
+  if (i < nw) {
+    zc* v0 = &Vw[i + q * NW];
+    const zc x = v0[0], y = v0[NW];
+    v0[0] = zadd(zscale(x, c), zmul(sc, y));
+    v0[NW] = zsub(zscale(y, c), zmul(s, x));
+  }
+  if (lane == 0) {
+    Tw[q + q * NW] = t22;
+    Tw[(q + 1) + (q + 1) * NW] = t11;
+    Tw[(q + 1) + q * NW] = zmake(0, 0);
+  }
+  wave_sync();
+}
+
+// The deflation test of the AED window (zlaqr3): from the bottom, an
+// eigenvalue whose spike entry |s| |V(0, j)| is negligible against its own
+// size deflates; one that is not is moved up to the top of the undeflated
+// entries.  Returns ns, the undeflated count: they occupy 0 .. ns-1, the
+// deflated ones ns .. nw-1.
+template <int NW>
+__device__ int aed_deflate(zc* Tw, zc* Vw, int nw, zc spike, double smlnum, int lane) {
+  const double s1 = zabs1(spike);
+  int ns = nw, ilst = 0;
+  while (ns > ilst) {
+    const int j = ns - 1;
+    double foo = zabs1(Tw[j + j * NW]);
+    if (foo == 0.0) foo = s1;
+    if (s1 * zabs1(Vw[j * NW]) <= fmax(smlnum, DBL_EPSILON * foo)) {
+      --ns;
+      continue;
+    }
+    for (int q = j - 1; q >= ilst; --q) swap_adjacent<NW>(Tw, Vw, nw, q, lane);
+    ++ilst;
+  }
+  return ns;
+}
+
+// Householder reflector (LAPACK's zlarfg convention) of the len entries
+// x[0 .. len-1] (LDS, one per lane): H = I - tau v v^H, v[0] = 1, with
+// H^H x = beta e1.  v goes to hv[0 .. len-1]; returns tau, beta in *beta.
+__device__ zc house(const zc* x, int len, zc* hv, zc* beta, int lane) {
+  const zc alpha = x[0];
+  double xn = (lane >= 1 && lane < len) ? zabs2(x[lane]) : 0.0;
+  for (int off = 32; off > 0; off >>= 1) xn += __shfl_xor(xn, off);
+  wave_sync();
+  if (xn == 0.0 && alpha.y == 0.0) {
+    if (lane < len) hv[lane] = lane == 0 ? zmake(1, 0) : zmake(0, 0);
+    *beta = alpha;
+    wave_sync();
+    return zmake(0, 0);
+  }
+  const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn);
+  const double bt = alpha.x >= 0 ? -nrm : nrm;
+  const zc tau = zmake((bt - alpha.x) / bt, -alpha.y / bt);
+  const zc scl = zinv(zsub(alpha, zmake(bt, 0)));
+  if (lane < len) hv[lane] = lane == 0 ? zmake(1, 0) : zmul(x[lane], scl);
+  *beta = zmake(bt, 0);
+  wave_sync();
+  return tau;
+}
+
+// Rows r0 .. r0+len-1 of Tw, columns c0 .. nw-1: Tw <- H^H Tw.
+template <int NW>
+__device__ void house_left(zc* Tw, int nw, int r0, int len, int c0, const zc* hv, zc tau,
+                           int lane) {
+  const int j = c0 + lane;
+  if (j < nw) {
+    zc w = zmake(0, 0);
+    for (int i = 0; i < len; ++i) w = zadd(w, zmul(zconj(hv[i]), Tw[(r0 + i) + j * NW]));
+    w = zmul(w, zconj(tau));
+    for (int i = 0; i < len; ++i)
+      Tw[(r0 + i) + j * NW] = zsub(Tw[(r0 + i) + j * NW], zmul(hv[i], w));
+  }
+  wave_sync();
+}
+
+// Columns c0 .. c0+len-1 of M (rows 0 .. nw-1): M <- M H.
+template <int NW>
+__device__ void house_right(zc* M, int nw, int c0, int len, const zc* hv, zc tau, int lane) {
+  const int i = lane;
+  if (i < nw) {
+    zc w = zmake(0, 0);
+    for (int j = 0; j < len; ++j) w = zadd(w, zmul(M[i + (c0 + j) * NW], hv[j]));
+    w = zmul(w, tau);
+    for (int j = 0; j < len; ++j)
+      M[i + (c0 + j) * NW] = zsub(M[i + (c0 + j) * NW], zmul(w, zconj(hv[j])));
+  }
+  wave_sync();
+}
+
+// Back to Hessenberg form after the AED: the spike sv[0 .. ns-1] (the column
+// left of the window) and the undeflated block Tw[0:ns, 0:ns].  A reflector
+// folds the spike into its first entry (returned: the new H(kwtop,
+// kwtop-1)), then reflectors reduce Tw[0:ns, 0:ns] to Hessenberg form; all of
+// them act on the whole window and accumulate into Vw.
+template <int NW>
+__device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns, int lane) {
+  zc beta;
+  zc tau = house(sv, ns, hv, &beta, lane);
+  house_left<NW>(Tw, nw, 0, ns, 0, hv, tau, lane);
+  house_right<NW>(Tw, nw, 0, ns, hv, tau, lane);
+  house_right<NW>(Vw, nw, 0, ns, hv, tau, lane);
+  const zc spike = beta;
+  for (int k = 0; k + 2 < ns; ++k) {
+    const int len = ns - k - 1;
+    if (lane < len) xv[lane] = Tw[(k + 1 + lane) + k * NW];
+    wave_sync();
+    zc bk;
+    tau = house(xv, len, hv, &bk, lane);
+    if (lane < len) Tw[(k + 1 + lane) + k * NW] = lane == 0 ? bk : zmake(0, 0);
+    wave_sync();
+    house_left<NW>(Tw, nw, k + 1, len, k + 1, hv, tau, lane);
+    house_right<NW>(Tw, nw, k + 1, len, hv, tau, lane);
+    house_right<NW>(Vw, nw, k + 1, len, hv, tau, lane);
+  }
+  return spike;
+}
+
+// One workgroup per matrix: multishift QR with aggressive early deflation.
+// Each pass: the AED step on the trailing NW x NW window of the active block
+// (or the whole block once it fits: its Schur form is computed in LDS and the
+// block is done), then, unless it deflated more than 14% of the window, a
+// pass of nch <= NC single-shift sweeps with the bottom undeflated window
+// eigenvalues as shifts, chained GAP = 2 positions apart (chain c rotates at
+// k - c GAP at step t while chain 0 is at k), chased through the LDS band and
+// applied to the strips exactly as in schur_pipe_kernel, whose pipelining,
+// band layout and strip geometry this kernel shares (with nch chains in
+// place of two).  Chains that trail by GAP positions act on disjoint rows and
+// columns, and a row rotation commutes with a column rotation of another
+// chain, so a pass equals its sweeps one after the other, which is how the
+// rotation log records them.  The dense V of every AED step goes to the
+// block log, ordered with the sweeps (pos = sweeps logged before it).
+template <int R, int NT, int NC, int NW>
+__global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, int* info,
+                                                      int maxit_per_row, QrLog log, BlockLog blog) {
+  static_assert(GAP_MS * (NC - 1) + 1 <= R, "chains must fit behind one window step");
+  static_assert(NW <= 64 && 2 * R <= NW + 1, "AED window: one lane per row, passes beyond it");
+  constexpr int GAP = GAP_MS, GP = GAP * (NC - 1);
+  constexpr int RS = R + GP + 1;   // rows spanned by a window's rotations
+  constexpr int NR = NC * R;       // rotation entries per window
+  constexpr int LDB = R + GP + 3;  // band side
+  constexpr int FS = R + 1;        // side of a band's fresh block, at most
+  constexpr int PF = (FS * FS + 63) / 64;
+  constexpr int SW = NT / 64 - 1;  // strip waves
+  constexpr int JR = 2 * R + GP + 1, JL = (R * JR + 63) / 64;  // J column rows; loads per lane
+  constexpr int JS = 2 * R - 1, JSL = (R * JS + 63) / 64;      // J rows stored per step
+  constexpr int BL = (LDB * LDB + 63) / 64;
+  constexpr int HALF = 64 / NC;  // lanes per chain in the chase
+  __shared__ zc band[2][LDB * LDB];
+  // each strip wave's transpose tile; between passes the AED window, its
+  // Schur vectors and two reflector vectors
+  constexpr int TILE = SW * 64 * RS, AEDL = 2 * NW * NW + 3 * NW;
+  __shared__ zc stile[TILE > AEDL ? TILE : AEDL];
+  __shared__ zc jcol[R * JR];         // wave 0: column l of J(w) at l * JR
+  __shared__ double rc[2][NR];        // rotations of the current and the previous window
+  __shared__ zc rs[2][NR];
+  __shared__ int s_il;
+  __shared__ zc s_mu[NC];
+  __shared__ int s_aed[3];
+  __shared__ zc s_spike;
+  const int b = blockIdx.x;
+  zc* T = Tm.p + (long long)b * Tm.stride;
+  const QrLogM L = qr_log_of(log, b);
+  const long long ld = Tm.ld;
+  const unsigned tbytes = (unsigned)(ld * n * 16);  // below 2^31 (checked by the host)
+  const __amdgpu_buffer_rsrc_t trs = matrix_rsrc(T, tbytes);
+  auto off_of = [&](int i, int j) { return (unsigned)(i + (long long)j * ld) * 16u; };
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double eps = DBL_EPSILON;
+  const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
+  int* st = log.state + 4 * b;
+  int iu = st[0], iter = st[1], nsw = 0, used = 0, nblk = 0;
+  const double smlnum = DBL_MIN * ((double)n / DBL_EPSILON);
+  zc* const Tw = stile;
+  zc* const Vw = stile + NW * NW;
+  zc* const svw = Vw + NW * NW;
+  zc* const hvw = svw + NW;
+  zc* const xvw = hvw + NW;
+  zc* const BV = blog.V + (long long)b * blog.bcap * NW * NW;
+  long long total = st[2];
+  if (st[3] != QR_RUNNING) {
+    if (tid == 0) {
+      log.nsw[b] = 0;
+      blog.nblk[b] = 0;
+    }
+    return;
+  }
+  bool failed = false, stopped = false;
+  for (;;) {
+    if (iu <= 0 || failed) break;
+    if (tid == 0) s_il = 0;
+    __syncthreads();
+    for (int k = iu - tid; k >= 1; k -= NT) {
+      const double sub = zabs1(T[k + (k - 1) * ld]);
+      const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
+      if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
+    }
+    __syncthreads();
+    const int il = __builtin_amdgcn_readfirstlane(s_il);
+    if (il == iu) {
+      if (tid == 0) T[iu + (iu - 1) * ld] = zmake(0, 0);
+      --iu;
+      iter = 0;
+      __syncthreads();
+      continue;
+    }
+    if (nblk + 1 > blog.bcap || nsw + 1 > log.scap ||
+        (long long)used + (iu - il) > log.limit) {
+      // a log is full (no room for the AED block and one sweep after it):
+      // stop here; the logs are applied and the resumed kernel carries on.
+      // A fresh launch always has room for both, so every launch makes
+      // progress.
+      stopped = true;
+      break;
+    }
+    // ---- aggressive early deflation on the trailing window (the whole
+    // active block once it fits one)
+    const int mblk = iu - il + 1;
+    const bool whole = mblk <= NW;
+    const int aw = whole ? mblk : NW;
+    const int kwtop = iu - aw + 1;
+    if (tid == 0 && il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
+    for (int idx = tid; idx < aw * aw; idx += NT) {
+      const int i = idx % aw, j = idx / aw;
+      Tw[i + j * NW] = i <= j + 1 ? T[(kwtop + i) + (long long)(kwtop + j) * ld] : zmake(0, 0);
+      Vw[i + j * NW] = zmake(i == j ? 1.0 : 0.0, 0.0);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      // The window's own QR gets LAPACK zlahqr's budget, 30 max(10, nw)
+      // sweeps; a finished block's sweeps also count against the matrix's
+      // limit (maxit_per_row * n), an AED window's do not: when the window
+      // does not converge it deflates nothing, and its diagonal still serves
+      // as shifts
+      int sw = 0;
+      const long long own = 30LL * max(10, aw);
+      const bool conv =
+          small_schur<NW>(Tw, Vw, aw, lane, whole ? min(own, maxit - total) : own, sw);
+      int ns = aw;
+      zc spike = zmake(0, 0);
+      if (conv && whole) {
+        ns = 0;
+      } else if (!whole) {
+        spike = T[kwtop + (long long)(kwtop - 1) * ld];
+        if (conv) ns = aed_deflate<NW>(Tw, Vw, aw, spike, smlnum, lane);
+        // the bottom NC undeflated eigenvalues of the window are the shifts
+        const int take = min(NC, ns);
+        if (lane < take) s_mu[lane] = Tw[(ns - take + lane) * (NW + 1)];
+        if (ns > 0 && lane < ns) svw[lane] = zmul(spike, zconj(Vw[lane * NW]));
+        wave_sync();
+        if (ns > 1)
+          spike = aed_rehess<NW>(Tw, Vw, svw, hvw, xvw, aw, ns, lane);
+        else
+          spike = ns == 1 ? svw[0] : zmake(0, 0);
+      }
+      if (lane == 0) {
+        s_aed[0] = ns;
+        s_aed[1] = whole ? sw : 0;
+        s_aed[2] = conv || !whole ? 1 : 0;
+        s_spike = spike;
+      }
+    }
+    __syncthreads();
+    const int ns = __builtin_amdgcn_readfirstlane(s_aed[0]);
+    total += __builtin_amdgcn_readfirstlane(s_aed[1]);
+    if (!s_aed[2]) {
+      // the sweep budget ran out on a finished block
+      if (tid == 0 && info) info[b] = 1;
+      failed = true;
+      continue;
+    }
+    if (!whole) {
+      // the window and the spike back into the active block; the rows of the
+      // active block above the window times V
+      for (int idx = tid; idx < aw * aw; idx += NT) {
+        const int i = idx % aw, j = idx / aw;
+        if (i <= j + 1) T[(kwtop + i) + (long long)(kwtop + j) * ld] = Tw[i + j * NW];
+      }
+      if (tid == 0) T[kwtop + (long long)(kwtop - 1) * ld] = s_spike;
+      for (int r = il + tid; r < kwtop; r += NT) {
+        zc* trow = T + r + (long long)kwtop * ld;
+        zc x[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) x[q] = trow[q * ld];
+        for (int j = 0; j < NW; ++j) {
+          zc acc = zmake(0, 0);
+#pragma unroll
+          for (int q = 0; q < NW; ++q) acc = zadd(acc, zmul(x[q], Vw[q + j * NW]));
+          trow[j * ld] = acc;
+        }
+      }
+    }
+    // log V for the Schur vectors
+    for (int idx = tid; idx < NW * NW; idx += NT)
+      BV[(long long)nblk * NW * NW + idx] = Vw[idx];
+    if (tid == 0) {
+      blog.pos[(long long)b * blog.bcap + nblk] = nsw;
+      blog.k0[(long long)b * blog.bcap + nblk] = kwtop;
+      blog.nw[(long long)b * blog.bcap + nblk] = aw;
+    }
+    ++nblk;
+    __syncthreads();
+    if (whole) {
+      iu = il - 1;
+      iter = 0;
+      continue;
+    }
+    const int nd = aw - ns;
+    iu = kwtop + ns - 1;
+    iter = nd > 0 ? 0 : iter + 1;
+    // many deflations: look at the next window before sweeping (zlaqr0's
+    // nibble point, 14%); a block that now fits the window is finished there
+    if ((nd > 0 && 100 * nd > 14 * aw) || ns == 0 || iu - il + 1 <= NW) continue;
+    // as many chains as shifts, and as the rotation log still holds (at least
+    // one: the check above left room for a sweep over the larger block)
+    const int nch = (int)min((long long)min(NC, ns),
+                             min((long long)(log.scap - nsw), (log.limit - used) / (iu - il)));
+    total += nch;
+    if (total > maxit) {
+      if (tid == 0 && info) info[b] = 1;
+      failed = true;
+      continue;
+    }
+    if (tid == 0) {
+      if (iter > 0 && iter % 6 == 0) {
+        // no deflation for six passes: exceptional shifts off the bottom
+        // diagonal entries
+        for (int c = 0; c < nch; ++c) {
+          const int k = iu - c;
+          s_mu[c] = zadd(T[k + (long long)k * ld],
+                         zmake(0.75 * fabs(T[k + (long long)(k - 1) * ld].x), 0));
+        }
+      }
+      for (int c = 0; c < nch; ++c) {
+        L.il[nsw + c] = il;
+        L.iu[nsw + c] = iu;
+        L.off[nsw + c] = used + c * (iu - il);
+      }
+    }
+    __syncthreads();
+    const int nw = (iu - il + GAP * (nch - 1) + R - 1) / R;
+    auto wk0 = [&](int x) { return il + x * R; };
+    auto wce = [&](int x) { return x < nw ? min(il + x * R + R + 1, iu) : iu; };
+    auto wb0 = [&](int x) { return x > 0 ? il + x * R - GP - 1 : il; };
+    // chain c's rotation at step t of the window starting at k0 is active
+    auto active = [&](int k0, int t, int c) {
+      const int kk = k0 + t - c * GAP;
+      return c < nch && kk >= il && kk <= iu - 1;
+    };
+    for (int w = 0; w <= nw; ++w) {
+      const bool cur = w < nw, prev = w > 0;
+      const int k0 = wk0(w), b0 = wb0(w), ce = wce(w), nb = ce - b0 + 1;
+      const int pk0 = prev ? wk0(w - 1) : 0, pb0 = prev ? wb0(w - 1) : 0;
+      const int pce = prev ? wce(w - 1) : 0;
+      const double* prc = rc[(w + 1) & 1];  // window w-1's rotations
+      const zc* prs = rs[(w + 1) & 1];
+      zc* cb = band[w & 1];
+      const zc* pbnd = band[(w + 1) & 1];
+      if (wave == 0) {
+        // loads for window w+1: J(w+1), rows k0 - R - GP + q
+        const int ce1 = wce(w + 1);
+        const int nj1 = cur ? ce1 - ce : 0;
+        const int jr0 = k0 - R - GP;
+        zc yv[JL];
+#pragma unroll
+        for (int i = 0; i < JL; ++i) {
+          const int e = lane + 64 * i, col = e / JR, row = jr0 + e % JR;
+          const bool ok = col < nj1 && row >= il && row <= iu;
+          yv[i] = buf_load(trs, ok ? off_of(row, ce + 1 + col) : tbytes);
+        }
+        // band w+1's fresh block: rows/columns ce .. ce1, less the corner
+        zc pf[PF];
+        const int m1 = ce1 - ce + 1;
+        if (w + 1 < nw) {
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            const int idx = lane + 64 * p, i = idx % m1, j = idx / m1;
+            pf[p] = buf_load(trs, idx < m1 * m1 && idx != 0 && i <= j + 1
+                                      ? off_of(ce + i, ce + j)
+                                      : tbytes);
+          }
+        }
+        if (prev) {
+          // band w-1 (upper Hessenberg part) and window w-1's rotations
+          const int pnb = pce - pb0 + 1;
+#pragma unroll
+          for (int p = 0; p < BL; ++p) {
+            const int idx = lane + 64 * p, i = idx % pnb, j = idx / pnb;
+            const bool ok = idx < pnb * pnb && i <= j + 1;
+            buf_store(trs, ok ? off_of(pb0 + i, pb0 + j) : tbytes, pbnd[ok ? i + j * LDB : 0]);
+          }
+          for (int e = lane; e < NR; e += 64) {
+            const int t = e / NC, c = e % NC;
+            if (active(pk0, t, c)) {
+              const int kk = pk0 + t - c * GAP;
+              L.c[used + c * (iu - il) + (kk - il)] = prc[e];
+              L.s[used + c * (iu - il) + (kk - il)] = prs[e];
+            }
+          }
+        }
+        if (cur && prev) {
+          // J(w): jcol row q is row k0 - 2R - GP + q; window w-1's rotations
+          // on q = R .. 2R + GP, rows q < 2R - 1 to memory, the rest (rows
+          // b0 .. k0) into band w
+          const int nj = ce - pce;
+          if (lane < nj) {
+            zc xr[RS];
+#pragma unroll
+            for (int i = 0; i < RS; ++i) xr[i] = jcol[lane * JR + R + i];
+            rotate_window<R, NC, GAP>(xr, false, prc, prs);
+#pragma unroll
+            for (int i = 0; i < R - 1; ++i) jcol[lane * JR + R + i] = xr[i];
+#pragma unroll
+            for (int i = R - 1; i < RS; ++i)
+              cb[(i - (R - 1)) + (pce + 1 + lane - b0) * LDB] = xr[i];
+          }
+#pragma unroll
+          for (int i = 0; i < JSL; ++i) {
+            const int e = lane + 64 * i, col = e / JS, q = e % JS, row = k0 - 2 * R - GP + q;
+            const bool ok = col < nj && row >= il;
+            buf_store(trs, ok ? off_of(row, pce + 1 + col) : tbytes, jcol[ok ? col * JR + q : 0]);
+          }
+          // overlap with band w-1: rows/columns b0 .. pce
+          const int no = pce - b0 + 1, o = b0 - pb0;
+          for (int idx = lane; idx < no * no; idx += 64) {
+            const int i = idx % no, j = idx / no;
+            cb[i + j * LDB] = pbnd[(o + i) + (o + j) * LDB];
+          }
+          // rows pce+1 .. ce of columns b0 .. pce-1 (below the subdiagonal)
+          const int mzr = ce - pce, mzc = pce - b0;
+          for (int idx = lane; idx < mzr * mzc; idx += 64)
+            cb[(pce + 1 - b0 + idx % mzr) + (idx / mzr) * LDB] = zmake(0, 0);
+        } else if (cur) {
+          for (int idx = lane; idx < nb * nb; idx += 64) {
+            const int i = idx % nb, j = idx / nb;
+            cb[i + j * LDB] =
+                i <= j + 1 ? T[(b0 + i) + (long long)(b0 + j) * ld] : zmake(0, 0);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (cur) {
+          double* crc = rc[w & 1];
+          zc* crs = rs[w & 1];
+          const int ch = lane / HALF, hl = lane % HALF;
+          for (int t = 0; t < R; ++t) {
+            // every lane of a chain computes that chain's rotation
+            const int kk = k0 + t - ch * GAP;
+            const bool act = active(k0, t, ch);
+            double c = 1.0;
+            zc sr = zmake(0, 0), r = zmake(0, 0);
+            if (act) {
+              zc f, g;
+              if (kk == il) {
+                f = zsub(cb[(kk - b0) + (kk - b0) * LDB], s_mu[ch]);
+                g = cb[(kk + 1 - b0) + (kk - b0) * LDB];
+              } else {
+                f = cb[(kk - b0) + (kk - 1 - b0) * LDB];
+                g = cb[(kk + 1 - b0) + (kk - 1 - b0) * LDB];
+              }
+              givens_rsq(f, g, c, sr, r);
+            }
+            if (hl == 0) {
+              if (act && kk > il) {
+                cb[(kk - b0) + (kk - 1 - b0) * LDB] = r;
+                cb[(kk + 1 - b0) + (kk - 1 - b0) * LDB] = zmake(0, 0);
+              }
+              crc[NC * t + ch] = c;
+              crs[NC * t + ch] = sr;
+            }
+            if (act) {
+              const zc sc = zconj(sr);
+              for (int j = kk + hl; j <= ce; j += HALF) {
+                zc* p0 = &cb[(kk - b0) + (j - b0) * LDB];
+                const zc x = p0[0], y = p0[1];
+                p0[0] = zadd(zscale(x, c), zmul(sr, y));
+                p0[1] = zsub(zscale(y, c), zmul(sc, x));
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (act) {
+              const zc sc = zconj(sr);
+              const int ie = min(kk + 2, iu);
+              for (int i = b0 + hl; i <= ie; i += HALF) {
+                zc* q0 = &cb[(i - b0) + (kk - b0) * LDB];
+                const zc x = q0[0], y = q0[LDB];
+                q0[0] = zadd(zscale(x, c), zmul(sc, y));
+                q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+          }
+        }
+        if (nj1 > 0) {
+          // J(w+1) into jcol; window w-1's rotations on its rows q = 0 .. RS-1
+#pragma unroll
+          for (int i = 0; i < JL; ++i)
+            if (lane + 64 * i < R * JR) jcol[lane + 64 * i] = yv[i];
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          if (prev && lane < nj1) {
+            zc xr[RS];
+#pragma unroll
+            for (int i = 0; i < RS; ++i) xr[i] = jcol[lane * JR + i];
+            rotate_window<R, NC, GAP>(xr, false, prc, prs);
+#pragma unroll
+            for (int i = 0; i < RS; ++i) jcol[lane * JR + i] = xr[i];
+          }
+        }
+        if (w + 1 < nw) {
+          // band w+1's fresh block (into band w-1's buffer, read for the last
+          // time above); its corner row/column ce sits at ce - b0(w+1)
+          zc* nbnd = band[(w + 1) & 1];
+          const int f0 = ce - wb0(w + 1);
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            const int idx = lane + 64 * p;
+            if (idx < m1 * m1 && idx != 0)
+              nbnd[(f0 + idx % m1) + (f0 + idx / m1) * LDB] = pf[p];
+          }
+        }
+      } else if (prev) {
+        // window w-1's rotations: chunks of 64 columns of the right strip
+        // (columns r0 .. iu) and of 64 rows of the top strip (rows il ..
+        // pb0-1); a segment covers rows/columns pk0 - GP .. pk0 + R
+        const int sw = wave - 1;
+        const int r0 = wce(w + 1) + 1;
+        const int nright = max(iu - r0 + 1, 0);
+        const int ntop = max(pb0 - il, 0);
+        const int nrc = (nright + 63) / 64, nch2 = nrc + (ntop + 63) / 64;
+        const int s0 = pk0 - GP;
+        zc* tile = stile + sw * (64 * RS);
+        for (int ch = sw; ch < nch2; ch += SW) {
+          zc v[RS];
+          unsigned off[RS];
+          const bool right = ch < nrc;
+          if (right) {
+            const int cb0 = r0 + ch * 64, ncol = min(64, iu - cb0 + 1);
+#pragma unroll
+            for (int i = 0; i < RS; ++i) {
+              const int e = lane + 64 * i, col = e / RS, row = s0 + e % RS;
+              off[i] = col < ncol && row >= il && row <= iu ? off_of(row, cb0 + col) : tbytes;
+            }
+          } else {
+            const int row = il + (ch - nrc) * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < RS; ++q) {
+              const int col = s0 + q;
+              off[q] = row < pb0 && col >= il && col <= iu ? off_of(row, col) : tbytes;
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < RS; ++i) v[i] = buf_load(trs, off[i]);
+          if (right) {
+#pragma unroll
+            for (int i = 0; i < RS; ++i) tile[lane + 64 * i] = v[i];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int q = 0; q < RS; ++q) v[q] = tile[lane * RS + q];
+          }
+          rotate_window<R, NC, GAP>(v, !right, prc, prs);
+          if (right) {
+#pragma unroll
+            for (int q = 0; q < RS; ++q) tile[lane * RS + q] = v[q];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int i = 0; i < RS; ++i) v[i] = tile[lane + 64 * i];
+          }
+#pragma unroll
+          for (int i = 0; i < RS; ++i) buf_store(trs, off[i], v[i]);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+      __syncthreads();
+    }
+    used += nch * (iu - il);
+    nsw += nch;
+  }
+  if (tid == 0) {
+    log.nsw[b] = nsw;
+    blog.nblk[b] = nblk;
+    st[0] = iu;
+    st[1] = iter;
+    st[2] = (int)total;
+    st[3] = stopped ? QR_RUNNING : failed ? QR_FAILED : QR_DONE;
+    if (stopped) atomicAdd(log.pending, 1);
+  }
+}
+
+
+// The logs of schur_ms_kernel applied to the Schur vectors: the logged
+// sweeps in groups of at most G (never across a block), and before sweep
+// pos[q] the dense block q (Z[:, k0 .. k0+nw-1] <- Z[:, k0 .. k0+nw-1] V),
+// each thread one row of Z.
+template <int G, int TC, int NT, int NW>
+__global__ __launch_bounds__(NT) void qr_apply_ms_kernel(int n, ZMat Zm, QrLog log,
+                                                         BlockLog blog) {
+  __shared__ double tc[G * TC];
+  __shared__ zc ts[G * TC];
+  __shared__ zc vb[NW * NW];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  const int row = blockIdx.x * NT + tid;
+  zc* zrow = Zm.p + (long long)b * Zm.stride + row;  // entries zrow[c * ld]
+  const QrLogK LK = qr_log_k(log, b);
+  const QrLogM L = qr_log_of(log, b);
+  const int nsw = ((KInt)log.nsw)[b];
+  const int nblk = ((KInt)blog.nblk)[b];
+  const KInt bpos = (KInt)(blog.pos + (long long)b * blog.bcap);
+  const KInt bk0 = (KInt)(blog.k0 + (long long)b * blog.bcap);
+  const KInt bnw = (KInt)(blog.nw + (long long)b * blog.bcap);
+  const zc* BV = blog.V + (long long)b * blog.bcap * NW * NW;
+  int g0 = 0;
+  for (int q = 0; q <= nblk; ++q) {
+    const int end = q < nblk ? bpos[q] : nsw;
+    for (; g0 < end; g0 += G) {
+      const SweepGroup<G> g = load_group<G>(LK, g0, min(G, end - g0));
+      const int t0 = (g.tmin / (2 * G)) * (2 * G);
+      for (int T0 = t0; T0 <= g.tmax; T0 += TC) {
+        const int T1 = min(T0 + TC - 1, g.tmax);
+        __syncthreads();
+        fill_table<G, TC>(g, L, T0, tc, ts, tid, NT);
+        __syncthreads();
+        if (row < n) apply_chunk_strip<G, TC>(g, T0, T1, zrow, Zm.ld, tc, ts);
+      }
+    }
+    g0 = end;
+    if (q == nblk) break;
+    const int k0 = bk0[q], aw = bnw[q];
+    __syncthreads();
+    for (int e = tid; e < NW * NW; e += NT) {
+      const int i = e % NW, j = e / NW;
+      vb[e] = i < aw && j < aw ? BV[(long long)q * NW * NW + e] : zmake(0, 0);
+    }
+    __syncthreads();
+    if (row < n) {
+      zc x[NW];
+#pragma unroll
+      for (int i = 0; i < NW; ++i) x[i] = i < aw ? zrow[(long long)(k0 + i) * Zm.ld] : zmake(0, 0);
+      for (int j = 0; j < aw; ++j) {
+        zc acc = zmake(0, 0);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) acc = zadd(acc, zmul(x[i], vb[i + j * NW]));
+        zrow[(long long)(k0 + j) * Zm.ld] = acc;
+      }
+    }
+  }
+}
+
 __global__ void qr_state_init_kernel(int n, int batch, int* state, int* pending) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b == 0) *pending = 0;
@@ -1506,23 +2307,34 @@ namespace {
 // stops and resumes after its log is applied.
 long long qr_log_default(int n) { return (long long)n * n + 64; }
 int qr_log_sweeps(int n) { return 8 * n + 64; }
+// The multishift iteration: windows of MS_R steps, MS_NC chains, AED windows
+// of MS_NW; room for n/3 + 32 AED blocks per matrix and launch (a C2 layer
+// matrix, n = 882, takes about 200).
+constexpr int MS_R = 16, MS_NC = 8, MS_NW = 32;
+int qr_block_cap(int n) { return n / 3 + 32; }
 }  // namespace
 
 size_t schur_ws_bytes(int n, int batch) {
   const size_t cap = (size_t)qr_log_default(n), scap = (size_t)qr_log_sweeps(n);
+  const size_t bcap = (size_t)qr_block_cap(n);
   return (size_t)batch * (cap * (sizeof(double) + sizeof(zc)) + scap * 3 * sizeof(int) +
-                          5 * sizeof(int)) +
-         8 * 256;
+                          5 * sizeof(int) + bcap * (MS_NW * MS_NW * sizeof(zc) + 3 * sizeof(int)) +
+                          sizeof(int)) +
+         16 * 256;
 }
 
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                   int maxit_per_row, void* ws, long long log_cap) {
   if (n <= 0 || batch <= 0) return 0;
-  // RCWA_QR_SWEEP (read on every call) selects another form of the same
-  // iteration: =direct the unwindowed sweep, =window the windowed sweep that
-  // updates all of T and Z at once, =oneshot the logged sweep without the
-  // pipelined wave 0 (also used for matrices of 2^31 bytes or more, which
-  // the pipelined kernel's 32-bit buffer offsets cannot address).
+  // The default is the multishift iteration with aggressive early deflation
+  // (schur_ms_kernel).  RCWA_QR_SWEEP (read on every call) selects another
+  // form of the QR iteration, for tests and comparisons: =pipe the pipelined
+  // sweep with two chained sweeps and one Wilkinson shift per pass, =single
+  // the same with one sweep, =direct the unwindowed sweep, =window the
+  // windowed sweep that updates all of T and Z at once, =oneshot the logged
+  // sweep without the pipelined wave 0 (also used for matrices of 2^31 bytes
+  // or more, which the pipelined kernels' 32-bit buffer offsets cannot
+  // address).
   const char* mode_env = getenv("RCWA_QR_SWEEP");
   const char mode = mode_env ? mode_env[0] : 0;
   if (mode == 'd') return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
@@ -1541,6 +2353,13 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
     log.pending = (int*)align((uintptr_t)(log.state + 4 * (size_t)batch));
     log.cap = cap_full;
     log.scap = scap;
+    BlockLog blog;
+    blog.bcap = qr_block_cap(n);
+    blog.V = (zc*)align((uintptr_t)(log.pending + 1));
+    blog.pos = (int*)align((uintptr_t)(blog.V + (size_t)batch * blog.bcap * MS_NW * MS_NW));
+    blog.k0 = (int*)align((uintptr_t)(blog.pos + (size_t)batch * blog.bcap));
+    blog.nw = (int*)align((uintptr_t)(blog.k0 + (size_t)batch * blog.bcap));
+    blog.nblk = (int*)align((uintptr_t)(blog.nw + (size_t)batch * blog.bcap));
     // a smaller limit (at least one sweep's worth) only makes the sweep kernel
     // stop for its log to be applied more often
     log.limit = log_cap > 0 ? std::max<long long>(std::min(log_cap, cap_full), n) : cap_full;
@@ -1551,23 +2370,33 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
     // Each round: sweeps until done or log full, then the log applied.  A
     // round makes progress on every pending matrix (at least one sweep), and
     // the sweep limit bounds the sweeps, so the loop ends.
+    const bool huge = (long long)T.ld * n * 16 >= (1LL << 31);
+    const bool ms = !huge && mode != 'o' && mode != 's' && mode != 'p';
     for (;;) {
-      if (mode == 'o' || (long long)T.ld * n * 16 >= (1LL << 31))
+      if (mode == 'o' || huge)
         hipLaunchKernelGGL((schur_defer_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z,
                            info, maxit_per_row, log);
       else if (mode == 's')
         hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 1>), dim3(batch), dim3(256), 0, s, n, T,
                            Z, info, maxit_per_row, log);
-      else
+      else if (mode == 'p')
         hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 2>), dim3(batch), dim3(256), 0, s, n, T,
                            Z, info, maxit_per_row, log);
+      else
+        hipLaunchKernelGGL((schur_ms_kernel<MS_R, 256, MS_NC, MS_NW>), dim3(batch), dim3(256), 0,
+                           s, n, T, Z, info, maxit_per_row, log, blog);
       if ((e = (int)hipGetLastError())) return e;
       // the log reaches the Schur vectors only: the sweeps keep the active
       // block of T up to date, and eig_batched forms the whole Schur form
       // afterwards as Z^H A Z
-      hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
-                         dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, Z,
-                         log);
+      if (ms)
+        hipLaunchKernelGGL((qr_apply_ms_kernel<APPLY_G, APPLY_TC, APPLY_NT, MS_NW>),
+                           dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, Z,
+                           log, blog);
+      else
+        hipLaunchKernelGGL((qr_apply_kernel<APPLY_G, APPLY_TC, APPLY_NT>),
+                           dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, Z,
+                           log);
       if ((e = (int)hipGetLastError())) return e;
       int pending = 0;
       if ((e = (int)hipMemcpyAsync(&pending, log.pending, sizeof(int), hipMemcpyDeviceToHost, s)))

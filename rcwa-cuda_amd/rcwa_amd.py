@@ -620,7 +620,8 @@ class Server:
         _check(rc)
         tot = np.array([[res[j].R_tot, res[j].T_tot] for j in range(npol)])
         info_d = {"t_submit": info.t_submit, "t_start": info.t_start,
-                  "t_finish": info.t_finish, "batch_size": info.batch_size}
+                  "t_finish": info.t_finish, "batch_size": info.batch_size,
+                  "cond": max(res[j].cond for j in range(npol))}
         out = (R.reshape(npol, P, Q).transpose(0, 2, 1), T.reshape(npol, P, Q).transpose(0, 2, 1),
                tot, info_d)
         if fields:
