@@ -162,9 +162,8 @@ def serve_bench(args):
     """Serving form on a prefix of the seeded Poisson stream (mean 2000
     req/s, harmonics {7,11,15,21,25}^2, 1-30 layers, 256x256 grids): each
     request is submitted at its arrival time.  The plain per-call path's rate
-    is measured on the first --serve-percall requests of the same stream, run
-    one per call, which also checks that each of them gives the same result
-    alone as in the mix."""
+    is measured on the same requests, run one per call, which also checks
+    that each of them gives the same result alone as in the mix."""
     import threading
     import rcwa_amd as G
     import workloads as W
@@ -192,7 +191,7 @@ def serve_bench(args):
     first = min(r[3]["t_submit"] for r in res)
     last = max(r[3]["t_finish"] for r in res)
     served = len(res) / (last - first)
-    nsample = min(len(stream), args.serve_percall)
+    nsample = len(stream)
     ctx = G.Context(0)
     t1 = time.perf_counter()
     alone = []
@@ -256,10 +255,10 @@ def main():
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--planner", action="store_true", help="also time the sweep-plan form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--refine", type=int, default=None,
+                    help="fold refinement level (rcwa_ctx_set_refine; default: the library's)")
     ap.add_argument("--c5", action="store_true",
                     help="also time the 41x41-harmonic, 50-layer single structure")
-    ap.add_argument("--serve-percall", type=int, default=24,
-                    help="requests of the stream timed one per call")
     ap.add_argument("--serve", type=int, default=0,
                     help="also run the serving form on this many requests of the stream")
     ap.add_argument("--cpu-sample", choices=["layer", "star", "both"], help=argparse.SUPPRESS)
@@ -313,6 +312,8 @@ def main():
     hip.sync()
 
     ctx = G.Context(0)
+    if args.refine is not None:
+        ctx.set_refine(args.refine)
 
     def run(chunk, precision=G.RCWA_FP64):
         sel = probs[chunk * args.lams:(chunk + 1) * args.lams]

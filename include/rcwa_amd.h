@@ -141,6 +141,14 @@ rcwa_status rcwa_ctx_reset_stats(rcwa_ctx* ctx);
 /* Upper bound on the number of S-matrices batched into one chunk (0 = as
  * many as the HBM budget allows). */
 rcwa_status rcwa_ctx_set_chunk(rcwa_ctx* ctx, int max_chunk);
+/* Iterative refinement of the solves with the explicit inverse in the
+ * Redheffer fold, (I - S11 P22)^-1 (rcwa.cpp:602-606 solves with LU): one
+ * step Y += F (B - M Y).  Level 0 none, 1 the source columns the
+ * efficiencies use, 2 (default) also the n x n solve that carries the fold
+ * forward.
+ * Stacks whose layer S-matrices have large entries make M ill-conditioned;
+ * there the refined solves keep R + T = 1 for lossless stacks. */
+rcwa_status rcwa_ctx_set_refine(rcwa_ctx* ctx, int level);
 /* Per-phase eigensolver timing with HIP events (adds a synchronisation). */
 rcwa_status rcwa_ctx_set_timing(rcwa_ctx* ctx, int on);
 

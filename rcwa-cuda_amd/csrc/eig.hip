@@ -987,6 +987,30 @@ __device__ __forceinline__ void rotate_window(zc* x, bool colform, const double*
     }
 }
 
+// rotate_window for many chains (the multishift kernel): the same
+// rotations in the same order, with a scheduling barrier between steps so
+// that the NC rotations of a step are loaded from LDS when that step runs
+// instead of all R NC of them ahead (which would hold 3 R NC registers).
+template <int R, int NC, int GAP>
+__device__ __forceinline__ void rotate_window_staged(zc* x, bool colform, const double* rc,
+                                                     const zc* rs) {
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int e = NC * t + c, o = GAP * (NC - 1 - c) + t;
+      const double cs = rc[e];
+      zc s = rs[e];
+      if (colform) s.y = -s.y;
+      const zc a = x[o], b = x[o + 1];
+      x[o] = zadd(zscale(a, cs), zmul(s, b));
+      x[o + 1] = zsub(zscale(b, cs), zmul(zconj(s), a));
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
 // One workgroup per matrix: the iteration of schur_defer_kernel (same shifts,
 // deflation and rotation log), with the bulge chase pipelined and, for
 // NC = 2, two sweeps chased together.
@@ -1414,7 +1438,7 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Complex Schur form of the nw x nw upper Hessenberg Tw (leading dimension
-// NW, nw <= 64) by single-shift QR sweeps, each rotation applied to the whole
+// NW + 1, nw <= 64) by single-shift QR sweeps, each rotation applied to the whole
 // window and accumulated into Vw (starts as the identity).  One wave; every
 // lane follows the same control flow from LDS values.  Wilkinson shifts with
 // the exceptional shifts of schur_pipe_kernel at the 10th and 20th sweep on
@@ -1422,21 +1446,22 @@ __device__ __forceinline__ void wave_sync() {
 // the sweeps done.
 template <int NW>
 __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, int& sweeps) {
+  constexpr int LDW = NW + 1;
   int hi = nw - 1, it = 0;
   const double eps = DBL_EPSILON;
   while (hi > 0) {
     const int k = hi - lane;
     bool neg = false;
     if (k >= 1) {
-      const double sub = zabs1(Tw[k + (k - 1) * NW]);
-      const double tst = zabs1(Tw[(k - 1) + (k - 1) * NW]) + zabs1(Tw[k + k * NW]);
+      const double sub = zabs1(Tw[k + (k - 1) * LDW]);
+      const double tst = zabs1(Tw[(k - 1) + (k - 1) * LDW]) + zabs1(Tw[k + k * LDW]);
       neg = sub <= eps * tst || sub < DBL_MIN;
     }
     const unsigned long long msk = __ballot(neg);
     const int lo = msk ? hi - (__ffsll((long long)msk) - 1) : 0;
     wave_sync();
     if (lo == hi) {
-      if (lane == 0) Tw[hi + (hi - 1) * NW] = zmake(0, 0);
+      if (lane == 0) Tw[hi + (hi - 1) * LDW] = zmake(0, 0);
       wave_sync();
       --hi;
       it = 0;
@@ -1445,37 +1470,36 @@ __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, 
     if (sweeps >= budget) return false;
     ++sweeps;
     ++it;
-    if (lo > 0 && lane == 0) Tw[lo + (lo - 1) * NW] = zmake(0, 0);
+    if (lo > 0 && lane == 0) Tw[lo + (lo - 1) * LDW] = zmake(0, 0);
     wave_sync();
     zc mu;
     if (it == 10)
-      mu = zadd(Tw[lo + lo * NW], zmake(0.75 * fabs(Tw[(lo + 1) + lo * NW].x), 0));
+      mu = zadd(Tw[lo + lo * LDW], zmake(0.75 * fabs(Tw[(lo + 1) + lo * LDW].x), 0));
     else if (it == 20)
-      mu = zadd(Tw[hi + hi * NW], zmake(0.75 * fabs(Tw[hi + (hi - 1) * NW].x), 0));
+      mu = zadd(Tw[hi + hi * LDW], zmake(0.75 * fabs(Tw[hi + (hi - 1) * LDW].x), 0));
     else
-      mu = wilkinson(Tw[(hi - 1) + (hi - 1) * NW], Tw[(hi - 1) + hi * NW], Tw[hi + (hi - 1) * NW],
-                     Tw[hi + hi * NW]);
+      mu = wilkinson(Tw[(hi - 1) + (hi - 1) * LDW], Tw[(hi - 1) + hi * LDW], Tw[hi + (hi - 1) * LDW],
+                     Tw[hi + hi * LDW]);
     for (int q = lo; q < hi; ++q) {
       zc f, g;
       if (q == lo) {
-        f = zsub(Tw[lo + lo * NW], mu);
-        g = Tw[(lo + 1) + lo * NW];
+        f = zsub(Tw[lo + lo * LDW], mu);
+        g = Tw[(lo + 1) + lo * LDW];
       } else {
-        f = Tw[q + (q - 1) * NW];
-        g = Tw[(q + 1) + (q - 1) * NW];
+        f = Tw[q + (q - 1) * LDW];
+        g = Tw[(q + 1) + (q - 1) * LDW];
       }
       double c;
       zc s, r;
-      givens(f, g, c, s, r);
-      wave_sync();
+      givens_rsq(f, g, c, s, r);
       if (q > lo && lane == 0) {
-        Tw[q + (q - 1) * NW] = r;
-        Tw[(q + 1) + (q - 1) * NW] = zmake(0, 0);
+        Tw[q + (q - 1) * LDW] = r;
+        Tw[(q + 1) + (q - 1) * LDW] = zmake(0, 0);
       }
       const zc sc = zconj(s);
       const int j = q + lane;  // row rotation: columns q .. nw-1
       if (j < nw) {
-        zc* p0 = &Tw[q + j * NW];
+        zc* p0 = &Tw[q + j * LDW];
         const zc x = p0[0], y = p0[1];
         p0[0] = zadd(zscale(x, c), zmul(s, y));
         p0[1] = zsub(zscale(y, c), zmul(sc, x));
@@ -1483,16 +1507,16 @@ __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, 
       wave_sync();
       const int i = lane;  // column rotation: rows 0 .. min(q + 2, hi), and Vw
       if (i <= min(q + 2, hi)) {
-        zc* q0 = &Tw[i + q * NW];
-        const zc x = q0[0], y = q0[NW];
+        zc* q0 = &Tw[i + q * LDW];
+        const zc x = q0[0], y = q0[LDW];
         q0[0] = zadd(zscale(x, c), zmul(sc, y));
-        q0[NW] = zsub(zscale(y, c), zmul(s, x));
+        q0[LDW] = zsub(zscale(y, c), zmul(s, x));
       }
       if (i < nw) {
-        zc* v0 = &Vw[i + q * NW];
-        const zc x = v0[0], y = v0[NW];
+        zc* v0 = &Vw[i + q * LDW];
+        const zc x = v0[0], y = v0[LDW];
         v0[0] = zadd(zscale(x, c), zmul(sc, y));
-        v0[NW] = zsub(zscale(y, c), zmul(s, x));
+        v0[LDW] = zsub(zscale(y, c), zmul(s, x));
       }
       wave_sync();
     }
@@ -1506,36 +1530,36 @@ __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, 
 // diagonal entries trade places and T(q, q+1) keeps its value.
 template <int NW>
 __device__ void swap_adjacent(zc* Tw, zc* Vw, int nw, int q, int lane) {
-  const zc t11 = Tw[q + q * NW], t22 = Tw[(q + 1) + (q + 1) * NW];
+  constexpr int LDW = NW + 1;
+  const zc t11 = Tw[q + q * LDW], t22 = Tw[(q + 1) + (q + 1) * LDW];
   double c;
   zc s, r;
-  givens(Tw[q + (q + 1) * NW], zsub(t22, t11), c, s, r);
-  wave_sync();
+  givens_rsq(Tw[q + (q + 1) * LDW], zsub(t22, t11), c, s, r);
   const zc sc = zconj(s);
   const int j = q + 2 + lane;
   if (j < nw) {
-    zc* p0 = &Tw[q + j * NW];
+    zc* p0 = &Tw[q + j * LDW];
     const zc x = p0[0], y = p0[1];
     p0[0] = zadd(zscale(x, c), zmul(s, y));
     p0[1] = zsub(zscale(y, c), zmul(sc, x));
   }
   const int i = lane;
   if (i < q) {
-    zc* q0 = &Tw[i + q * NW];
-    const zc x = q0[0], y = q0[NW];
+    zc* q0 = &Tw[i + q * LDW];
+    const zc x = q0[0], y = q0[LDW];
     q0[0] = zadd(zscale(x, c), zmul(sc, y));
-    q0[NW] = zsub(zscale(y, c), zmul(s, x));
+    q0[LDW] = zsub(zscale(y, c), zmul(s, x));
   }
   if (i < nw) {
-    zc* v0 = &Vw[i + q * NW];
-    const zc x = v0[0], y = v0[NW];
+    zc* v0 = &Vw[i + q * LDW];
+    const zc x = v0[0], y = v0[LDW];
     v0[0] = zadd(zscale(x, c), zmul(sc, y));
-    v0[NW] = zsub(zscale(y, c), zmul(s, x));
+    v0[LDW] = zsub(zscale(y, c), zmul(s, x));
   }
   if (lane == 0) {
-    Tw[q + q * NW] = t22;
-    Tw[(q + 1) + (q + 1) * NW] = t11;
-    Tw[(q + 1) + q * NW] = zmake(0, 0);
+    Tw[q + q * LDW] = t22;
+    Tw[(q + 1) + (q + 1) * LDW] = t11;
+    Tw[(q + 1) + q * LDW] = zmake(0, 0);
   }
   wave_sync();
 }
@@ -1547,13 +1571,14 @@ __device__ void swap_adjacent(zc* Tw, zc* Vw, int nw, int q, int lane) {
 // deflated ones ns .. nw-1.
 template <int NW>
 __device__ int aed_deflate(zc* Tw, zc* Vw, int nw, zc spike, double smlnum, int lane) {
+  constexpr int LDW = NW + 1;
   const double s1 = zabs1(spike);
   int ns = nw, ilst = 0;
   while (ns > ilst) {
     const int j = ns - 1;
-    double foo = zabs1(Tw[j + j * NW]);
+    double foo = zabs1(Tw[j + j * LDW]);
     if (foo == 0.0) foo = s1;
-    if (s1 * zabs1(Vw[j * NW]) <= fmax(smlnum, DBL_EPSILON * foo)) {
+    if (s1 * zabs1(Vw[j * LDW]) <= fmax(smlnum, DBL_EPSILON * foo)) {
       --ns;
       continue;
     }
@@ -1591,13 +1616,14 @@ __device__ zc house(const zc* x, int len, zc* hv, zc* beta, int lane) {
 template <int NW>
 __device__ void house_left(zc* Tw, int nw, int r0, int len, int c0, const zc* hv, zc tau,
                            int lane) {
+  constexpr int LDW = NW + 1;
   const int j = c0 + lane;
   if (j < nw) {
     zc w = zmake(0, 0);
-    for (int i = 0; i < len; ++i) w = zadd(w, zmul(zconj(hv[i]), Tw[(r0 + i) + j * NW]));
+    for (int i = 0; i < len; ++i) w = zadd(w, zmul(zconj(hv[i]), Tw[(r0 + i) + j * LDW]));
     w = zmul(w, zconj(tau));
     for (int i = 0; i < len; ++i)
-      Tw[(r0 + i) + j * NW] = zsub(Tw[(r0 + i) + j * NW], zmul(hv[i], w));
+      Tw[(r0 + i) + j * LDW] = zsub(Tw[(r0 + i) + j * LDW], zmul(hv[i], w));
   }
   wave_sync();
 }
@@ -1605,13 +1631,14 @@ __device__ void house_left(zc* Tw, int nw, int r0, int len, int c0, const zc* hv
 // Columns c0 .. c0+len-1 of M (rows 0 .. nw-1): M <- M H.
 template <int NW>
 __device__ void house_right(zc* M, int nw, int c0, int len, const zc* hv, zc tau, int lane) {
+  constexpr int LDW = NW + 1;
   const int i = lane;
   if (i < nw) {
     zc w = zmake(0, 0);
-    for (int j = 0; j < len; ++j) w = zadd(w, zmul(M[i + (c0 + j) * NW], hv[j]));
+    for (int j = 0; j < len; ++j) w = zadd(w, zmul(M[i + (c0 + j) * LDW], hv[j]));
     w = zmul(w, tau);
     for (int j = 0; j < len; ++j)
-      M[i + (c0 + j) * NW] = zsub(M[i + (c0 + j) * NW], zmul(w, zconj(hv[j])));
+      M[i + (c0 + j) * LDW] = zsub(M[i + (c0 + j) * LDW], zmul(w, zconj(hv[j])));
   }
   wave_sync();
 }
@@ -1623,6 +1650,7 @@ __device__ void house_right(zc* M, int nw, int c0, int len, const zc* hv, zc tau
 // them act on the whole window and accumulate into Vw.
 template <int NW>
 __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns, int lane) {
+  constexpr int LDW = NW + 1;
   zc beta;
   zc tau = house(sv, ns, hv, &beta, lane);
   house_left<NW>(Tw, nw, 0, ns, 0, hv, tau, lane);
@@ -1631,11 +1659,11 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
   const zc spike = beta;
   for (int k = 0; k + 2 < ns; ++k) {
     const int len = ns - k - 1;
-    if (lane < len) xv[lane] = Tw[(k + 1 + lane) + k * NW];
+    if (lane < len) xv[lane] = Tw[(k + 1 + lane) + k * LDW];
     wave_sync();
     zc bk;
     tau = house(xv, len, hv, &bk, lane);
-    if (lane < len) Tw[(k + 1 + lane) + k * NW] = lane == 0 ? bk : zmake(0, 0);
+    if (lane < len) Tw[(k + 1 + lane) + k * LDW] = lane == 0 ? bk : zmake(0, 0);
     wave_sync();
     house_left<NW>(Tw, nw, k + 1, len, k + 1, hv, tau, lane);
     house_right<NW>(Tw, nw, k + 1, len, hv, tau, lane);
@@ -1677,7 +1705,8 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
   __shared__ zc band[2][LDB * LDB];
   // each strip wave's transpose tile; between passes the AED window, its
   // Schur vectors and two reflector vectors
-  constexpr int TILE = SW * 64 * RS, AEDL = 2 * NW * NW + 3 * NW;
+  constexpr int LDW = NW + 1;  // leading dimension of the AED window in LDS
+  constexpr int TILE = SW * 64 * RS, AEDL = 2 * NW * LDW + 3 * NW;
   __shared__ zc stile[TILE > AEDL ? TILE : AEDL];
   __shared__ zc jcol[R * JR];         // wave 0: column l of J(w) at l * JR
   __shared__ double rc[2][NR];        // rotations of the current and the previous window
@@ -1700,8 +1729,8 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
   int iu = st[0], iter = st[1], nsw = 0, used = 0, nblk = 0;
   const double smlnum = DBL_MIN * ((double)n / DBL_EPSILON);
   zc* const Tw = stile;
-  zc* const Vw = stile + NW * NW;
-  zc* const svw = Vw + NW * NW;
+  zc* const Vw = stile + NW * LDW;
+  zc* const svw = Vw + NW * LDW;
   zc* const hvw = svw + NW;
   zc* const xvw = hvw + NW;
   zc* const BV = blog.V + (long long)b * blog.bcap * NW * NW;
@@ -1750,8 +1779,8 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
     if (tid == 0 && il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
     for (int idx = tid; idx < aw * aw; idx += NT) {
       const int i = idx % aw, j = idx / aw;
-      Tw[i + j * NW] = i <= j + 1 ? T[(kwtop + i) + (long long)(kwtop + j) * ld] : zmake(0, 0);
-      Vw[i + j * NW] = zmake(i == j ? 1.0 : 0.0, 0.0);
+      Tw[i + j * LDW] = i <= j + 1 ? T[(kwtop + i) + (long long)(kwtop + j) * ld] : zmake(0, 0);
+      Vw[i + j * LDW] = zmake(i == j ? 1.0 : 0.0, 0.0);
     }
     __syncthreads();
     if (wave == 0) {
@@ -1773,8 +1802,8 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
         if (conv) ns = aed_deflate<NW>(Tw, Vw, aw, spike, smlnum, lane);
         // the bottom NC undeflated eigenvalues of the window are the shifts
         const int take = min(NC, ns);
-        if (lane < take) s_mu[lane] = Tw[(ns - take + lane) * (NW + 1)];
-        if (ns > 0 && lane < ns) svw[lane] = zmul(spike, zconj(Vw[lane * NW]));
+        if (lane < take) s_mu[lane] = Tw[(ns - take + lane) * (LDW + 1)];
+        if (ns > 0 && lane < ns) svw[lane] = zmul(spike, zconj(Vw[lane * LDW]));
         wave_sync();
         if (ns > 1)
           spike = aed_rehess<NW>(Tw, Vw, svw, hvw, xvw, aw, ns, lane);
@@ -1802,7 +1831,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
       // active block above the window times V
       for (int idx = tid; idx < aw * aw; idx += NT) {
         const int i = idx % aw, j = idx / aw;
-        if (i <= j + 1) T[(kwtop + i) + (long long)(kwtop + j) * ld] = Tw[i + j * NW];
+        if (i <= j + 1) T[(kwtop + i) + (long long)(kwtop + j) * ld] = Tw[i + j * LDW];
       }
       if (tid == 0) T[kwtop + (long long)(kwtop - 1) * ld] = s_spike;
       for (int r = il + tid; r < kwtop; r += NT) {
@@ -1813,14 +1842,14 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
         for (int j = 0; j < NW; ++j) {
           zc acc = zmake(0, 0);
 #pragma unroll
-          for (int q = 0; q < NW; ++q) acc = zadd(acc, zmul(x[q], Vw[q + j * NW]));
+          for (int q = 0; q < NW; ++q) acc = zadd(acc, zmul(x[q], Vw[q + j * LDW]));
           trow[j * ld] = acc;
         }
       }
     }
     // log V for the Schur vectors
     for (int idx = tid; idx < NW * NW; idx += NT)
-      BV[(long long)nblk * NW * NW + idx] = Vw[idx];
+      BV[(long long)nblk * NW * NW + idx] = Vw[(idx % NW) + (idx / NW) * LDW];
     if (tid == 0) {
       blog.pos[(long long)b * blog.bcap + nblk] = nsw;
       blog.k0[(long long)b * blog.bcap + nblk] = kwtop;
@@ -1935,7 +1964,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
             zc xr[RS];
 #pragma unroll
             for (int i = 0; i < RS; ++i) xr[i] = jcol[lane * JR + R + i];
-            rotate_window<R, NC, GAP>(xr, false, prc, prs);
+            rotate_window_staged<R, NC, GAP>(xr, false, prc, prs);
 #pragma unroll
             for (int i = 0; i < R - 1; ++i) jcol[lane * JR + R + i] = xr[i];
 #pragma unroll
@@ -2032,7 +2061,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
             zc xr[RS];
 #pragma unroll
             for (int i = 0; i < RS; ++i) xr[i] = jcol[lane * JR + i];
-            rotate_window<R, NC, GAP>(xr, false, prc, prs);
+            rotate_window_staged<R, NC, GAP>(xr, false, prc, prs);
 #pragma unroll
             for (int i = 0; i < RS; ++i) jcol[lane * JR + i] = xr[i];
           }
@@ -2089,7 +2118,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
 #pragma unroll
             for (int q = 0; q < RS; ++q) v[q] = tile[lane * RS + q];
           }
-          rotate_window<R, NC, GAP>(v, !right, prc, prs);
+          rotate_window_staged<R, NC, GAP>(v, !right, prc, prs);
           if (right) {
 #pragma unroll
             for (int q = 0; q < RS; ++q) tile[lane * RS + q] = v[q];

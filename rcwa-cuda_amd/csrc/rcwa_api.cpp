@@ -404,6 +404,19 @@ inline cplx C(rcwa_c128 z) { return cplx(z.re, z.im); }
 
 }  // namespace
 
+rcwa_status validate_problem(const rcwa_problem& pr) {
+  if (rcwa_status st = check_device(pr.device, pr.params)) return st;
+  if (pr.nsrc <= 0 || !pr.src) return rcwa_set_error(RCWA_E_ARG, "problem without sources");
+  for (int i = 0; i < pr.nsrc; ++i)
+    if (rcwa_status st = check_source(pr.src[i])) return st;
+  return RCWA_OK;
+}
+
+rcwa_status check_wave_vectors(const rcwa_host::WaveVectors& wv, bool has_layers, int hx,
+                               int hy) {
+  return check_kz(wv, has_layers, hx, hy);
+}
+
 // HBM bytes of one chunk: G S-matrices with TL layers in total, layer
 // sub-batch Bl.
 size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool host_grids,
@@ -435,6 +448,102 @@ static size_t sweep_chunk_extra_bytes(int N, int L, int G, int M, int share, boo
   size_t z = 2 * (size_t)G * M;
   if (share >= 0) z += (mu_unit ? 1 : 2) * T1 * N2 + rows * L * N2;
   return z * sizeof(zc) + 2 * (rows * L + T1) * sizeof(int) + 6 * 256;
+}
+
+// ------------------------------------------------------------------ fold
+// The fold for the two source columns:
+// S_glob = S_ref * S_0 * ... * S_{L-1} * S_trn (Redheffer products,
+// rcwa.cpp:588-615), associated left to right.  The efficiencies need only
+// the columns of S_glob's S11 and S21 at the two source entries
+// (rcwa.cpp:512-514, 535, 553), so the running product P keeps P12 and P22
+// as matrices and only those columns a = P11 c, b = P21 c.  With
+// F = (I - X11 P22)^-1, one step P * X is
+//   a += P12 F X11 b,   b = X21 (b + P22 F X11 b),
+//   P12 = P12 F X12,    P22 = X22 + X21 P22 F X12
+// (the push-through identity (I - P22 X11)^-1 P22 = P22 F turns the
+// reference's second inverse into products): one inverse and five n^3
+// products per layer instead of two and ten, and the last step (S_trn)
+// needs only the columns.
+//
+// One step of iterative refinement for the solves with the explicit inverse
+// F of M = I - X11 P22 (Y = F B, then Y += F (B - M Y)): with layer
+// S-matrices of large entries M is ill-conditioned, and a product with an
+// explicit inverse is then much less accurate than a backward-stable solve
+// (the reference's LU solves, rcwa.cpp:602-606); the refined solve is as
+// accurate as one.  Level 1 refines the column solves (two n x 2 products),
+// level 2 also F X12 (two n^3 products).
+int Fold::gemm(int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D_, ZMat C_,
+               int bt) {
+  return on2 ? c->gemm2(M_, N_, K_, al, A_, B_, be, D_, C_, bt)
+             : c->gemm(M_, N_, K_, al, A_, B_, be, D_, C_, bt);
+}
+
+rcwa_status Fold::start(const zc* ends, long long estride, int cnt) {
+  const long long n2 = (long long)n * n;
+  const zc one = zmake(1, 0);
+  auto Mt = [&](zc* p) { return ZMat{p, n, n2}; };
+  auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  CK(zset_identity(fs, n, ZMat{Id, n, n2}, 1, one));
+  // P = S_ref: full S12, S22; columns of S11, S21
+  CK(expand_block_diag(fs, N, ends + 0 * N, estride, Mt(X11), cnt));
+  CK(expand_block_diag(fs, N, ends + 4 * (size_t)N, estride, Mt(P12), cnt));
+  CK(expand_block_diag(fs, N, ends + 8 * (size_t)N, estride, Mt(X21), cnt));
+  CK(expand_block_diag(fs, N, ends + 12 * (size_t)N, estride, Mt(P22), cnt));
+  CK(gather_cols(fs, n, CMt(X11), colidx, 2, va, cnt));
+  CK(gather_cols(fs, n, CMt(X21), colidx, 2, vb, cnt));
+  return RCWA_OK;
+}
+
+rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool columns_only) {
+  const long long n2 = (long long)n * n;
+  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const ZCMat none{nullptr, 0, 0};
+  const ZCMat ident{Id, n, 0};  // stride 0: the same identity for every product
+  auto Mt = [&](zc* p) { return ZMat{p, n, n2}; };
+  auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  auto Mv = [&](zc* p) { return ZMat{p, n, 2LL * n}; };
+  auto CMv = [&](const zc* p) { return ZCMat{p, n, 2LL * n}; };
+  zc* vw = vu + (size_t)G * 2 * n;
+  zc* vy = vw + (size_t)G * 2 * n;
+  const int refine = c->refine;
+  CK(gemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
+  if (refine > 0) CK(zcopy_batched(fs, n, n, CMt(Fm), Mt(Mf), na));
+  CK(gj_inverse_batched(fs, n, Mt(Fm), na, gj, info));
+  // the columns, with the old P12 and P22
+  CK(gemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
+  CK(gemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
+  if (refine > 0) {
+    CK(gemm(n, 2, n, mone, CMt(Mf), CMv(vw), one, CMv(vu), Mv(vy), na));
+    CK(gemm(n, 2, n, one, CMt(Fm), CMv(vy), one, CMv(vw), Mv(vw), na));
+  }
+  CK(gemm(n, 2, n, one, CMt(P12), CMv(vw), one, CMv(va), Mv(va), na));
+  CK(gemm(n, 2, n, one, CMt(P22), CMv(vw), one, CMv(vb), Mv(vy), na));
+  CK(gemm(n, 2, n, one, x21, CMv(vy), zero, none, Mv(vb), na));
+  if (columns_only) return RCWA_OK;
+  CK(gemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
+  if (refine > 1) {
+    CK(gemm(n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na));
+    CK(gemm(n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na));
+  }
+  CK(gemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
+  CK(gemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
+  CK(gemm(n, n, n, one, x21, CMt(Hm), one, x22, Mt(P22n), na));
+  if (na < G) {  // the products already past their last layer keep their P
+    CK(zcopy_batched(fs, n, n, CMt(P12 + (size_t)na * n2), Mt(P12n + (size_t)na * n2), G - na));
+    CK(zcopy_batched(fs, n, n, CMt(P22 + (size_t)na * n2), Mt(P22n + (size_t)na * n2), G - na));
+  }
+  std::swap(P12, P12n);
+  std::swap(P22, P22n);
+  return RCWA_OK;
+}
+
+rcwa_status Fold::finish(const zc* ends, long long estride, int cnt) {
+  const long long n2 = (long long)n * n;
+  auto Mt = [&](zc* p) { return ZMat{p, n, n2}; };
+  auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  CK(expand_block_diag(fs, N, ends + 16 * (size_t)N, estride, Mt(X11), cnt));
+  CK(expand_block_diag(fs, N, ends + 24 * (size_t)N, estride, Mt(X21), cnt));
+  return step(cnt, CMt(X11), CMt(X11), CMt(X21), CMt(X11), true);
 }
 
 // Computes the S-matrices and efficiencies of G jobs that share (hx, hy,
@@ -714,78 +823,37 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     CKH(hipStreamWaitEvent(to, e, 0));
     return RCWA_OK;
   };
-  auto fgemm = [&](int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D_,
-                   ZMat C_, int bt) {
-    return overlap ? c.gemm2(M_, N_, K_, al, A_, B_, be, D_, C_, bt)
-                   : c.gemm(M_, N_, K_, al, A_, B_, be, D_, C_, bt);
-  };
   if ((st = fence(s, fs)) != RCWA_OK) return st;
-  // The fold for the two source columns:
-  // S_glob = S_ref * S_0 * ... * S_{L-1} * S_trn (Redheffer products,
-  // rcwa.cpp:588-615), associated left to right.  The efficiencies need only
-  // the columns of S_glob's S11 and S21 at the two source entries
-  // (rcwa.cpp:512-514, 535, 553), so the running product P keeps P12 and P22
-  // as matrices and only those columns a = P11 c, b = P21 c.  With
-  // F = (I - X11 P22)^-1, one step P * X is
-  //   a += P12 F X11 b,   b = X21 (b + P22 F X11 b),
-  //   P12 = P12 F X12,    P22 = X22 + X21 P22 F X12
-  // (the push-through identity (I - P22 X11)^-1 P22 = P22 F turns the
-  // reference's second inverse into products): one inverse and five n^3
-  // products per layer instead of two and ten, and the last step (S_trn)
-  // needs only the columns.
-  zc* P12 = slot[0];
-  zc* P22 = slot[1];
-  zc* P12n = slot[2];
-  zc* P22n = slot[3];
-  zc* Fm = slot[4];
-  zc* Gm = slot[5];
-  zc* Hm = slot[6];
-  zc* Id = slot[7];
-  zc* vu = slot[8];  // three n x 2 column pairs per job (a slot holds G n^2 >= G 6n)
-  zc* vw = vu + (size_t)G * 2 * n;
-  zc* vy = vw + (size_t)G * 2 * n;
-  zc* X11 = slot[9];
-  zc* X21 = slot[10];
-  zc* va = dcols;  // S11 columns, then S21 columns: read back by the host
-  zc* vb = dcols + (size_t)G * 2 * n;
-  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
-  const ZCMat none{nullptr, 0, 0};
-  const ZCMat ident{Id, n, 0};  // stride 0: the same identity for every job
-  auto Mt = [&](zc* p) { return ZMat{p, n, n2}; };
+  // The fold for the two source columns (Fold: S_ref, then one step per
+  // layer, then S_trn).
+  Fold fold;
+  fold.c = &c;
+  fold.fs = fs;
+  fold.on2 = overlap;
+  fold.N = N;
+  fold.n = n;
+  fold.G = G;
+  fold.P12 = slot[0];
+  fold.P22 = slot[1];
+  fold.P12n = slot[2];
+  fold.P22n = slot[3];
+  fold.Fm = slot[4];
+  fold.Gm = slot[5];
+  fold.Hm = slot[6];
+  fold.Id = slot[7];
+  fold.vu = slot[8];  // three n x 2 column pairs per job (a slot holds G n^2 >= G 6n)
+  fold.X11 = slot[9];
+  fold.X21 = slot[10];
+  fold.Mf = slot[11];
+  fold.va = dcols;  // S11 columns, then S21 columns: read back by the host
+  fold.vb = dcols + (size_t)G * 2 * n;
+  fold.gj = gjf;
+  fold.info = info_sing;
+  fold.colidx = dcolidx;
+  if ((st = fold.start(dends, 32LL * N, G)) != RCWA_OK) return st;
   auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
-  auto Mv = [&](zc* p) { return ZMat{p, n, 2LL * n}; };
-  auto CMv = [&](const zc* p) { return ZCMat{p, n, 2LL * n}; };
-  CK(zset_identity(fs, n, ZMat{Id, n, n2}, 1, one));
-  // P = S_ref: full S12, S22; columns of S11, S21
-  CK(expand_block_diag(fs, N, dends + 0 * N, 32LL * N, Mt(X11), G));
-  CK(expand_block_diag(fs, N, dends + 4 * (size_t)N, 32LL * N, Mt(P12), G));
-  CK(expand_block_diag(fs, N, dends + 8 * (size_t)N, 32LL * N, Mt(X21), G));
-  CK(expand_block_diag(fs, N, dends + 12 * (size_t)N, 32LL * N, Mt(P22), G));
-  CK(gather_cols(fs, n, CMt(X11), dcolidx, 2, va, G));
-  CK(gather_cols(fs, n, CMt(X21), dcolidx, 2, vb, G));
-  // P <- P * X for the first na jobs
-  auto step = [&](int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22,
-                  bool columns_only) -> rcwa_status {
-    CK(fgemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
-    CK(gj_inverse_batched(fs, n, Mt(Fm), na, gjf, info_sing));
-    // the columns, with the old P12 and P22
-    CK(fgemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
-    CK(fgemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
-    CK(fgemm(n, 2, n, one, CMt(P12), CMv(vw), one, CMv(va), Mv(va), na));
-    CK(fgemm(n, 2, n, one, CMt(P22), CMv(vw), one, CMv(vb), Mv(vy), na));
-    CK(fgemm(n, 2, n, one, x21, CMv(vy), zero, none, Mv(vb), na));
-    if (columns_only) return RCWA_OK;
-    CK(fgemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
-    CK(fgemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
-    CK(fgemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
-    CK(fgemm(n, n, n, one, x21, CMt(Hm), one, x22, Mt(P22n), na));
-    if (na < G) {  // the jobs already past their last layer keep their P
-      CK(zcopy_batched(fs, n, n, CMt(P12 + (size_t)na * n2), Mt(P12n + (size_t)na * n2), G - na));
-      CK(zcopy_batched(fs, n, n, CMt(P22 + (size_t)na * n2), Mt(P22n + (size_t)na * n2), G - na));
-    }
-    std::swap(P12, P12n);
-    std::swap(P22, P22n);
-    return RCWA_OK;
+  auto step = [&](int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool columns_only) {
+    return fold.step(na, x11, x12, x21, x22, columns_only);
   };
   int lf = 0;  // next fold step
   // ---- SMatrixLayer for every layer (main.cpp:151)
@@ -824,10 +892,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   }
   CKH(hipEventRecord(c.ev[4], s));
   // * S_trn: only the columns
-  CK(expand_block_diag(fs, N, dends + 16 * (size_t)N, 32LL * N, Mt(X11), G));
-  CK(expand_block_diag(fs, N, dends + 24 * (size_t)N, 32LL * N, Mt(X21), G));
-  st = step(G, CMt(X11), CMt(X11), CMt(X21), CMt(X11), true);
-  if (st) return st;
+  if ((st = fold.finish(dends, 32LL * N, G)) != RCWA_OK) return st;
   if ((st = fence(fs, s)) != RCWA_OK) return st;
   CKH(hipEventRecord(c.ev[5], s));  // ms_fold: the fold's time not hidden by the layers
   // ---- S11 / S21 columns at the source entries (rcwa.cpp:512-514, 535, 553)
@@ -1096,6 +1161,12 @@ rcwa_status rcwa_ctx_set_eig_maxit(rcwa_ctx* ctx, int maxit) {
 rcwa_status rcwa_ctx_set_qr_log(rcwa_ctx* ctx, long long rotations) {
   if (!ctx || rotations < 0) return rcwa_set_error(RCWA_E_ARG, "bad argument");
   reinterpret_cast<Ctx*>(ctx)->qr_log = rotations;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_set_refine(rcwa_ctx* ctx, int level) {
+  if (!ctx || level < 0 || level > 2) return rcwa_set_error(RCWA_E_ARG, "bad argument");
+  reinterpret_cast<Ctx*>(ctx)->refine = level;
   return RCWA_OK;
 }
 

@@ -6,6 +6,7 @@
 
 #include "rcwa_amd.h"
 #include "common.hpp"
+#include "host_math.hpp"
 
 rcwa_status rcwa_set_error(rcwa_status s, const std::string& msg);
 
@@ -31,6 +32,7 @@ struct Ctx {
   int eig_maxit = 30;
   long long qr_log = 0;  // QR rotation-log capacity per matrix (0 = default)
   int max_chunk = 0;
+  int refine = 2;  // iterative refinement of the fold's solves (rcwa_ctx_set_refine)
   rcwa_precision precision = RCWA_FP64;
   rcwa_stats stats{};
   std::vector<std::complex<double>> last_cols;
@@ -90,6 +92,34 @@ rcwa_status star_product_device(Ctx& c, int n, int batch, const rcwa_c128* const
                                 const rcwa_c128* const* B_dev, rcwa_c128* const* S_dev, zc* t[5],
                                 void* gj, int* dinfo);
 
+// The Redheffer fold of G running products P = S_ref * S_0 * ... (contiguous
+// batches of n x n blocks, n = 2N), keeping P12, P22 and the S11/S21 columns
+// at the two source entries (va, vb: G x n x 2 each).  Work slots: P12n,
+// P22n, Fm, Gm, Hm, Mf (G n^2 each), Id (n^2), X11, X21 (G n^2), vu (3 G n
+// x 2).  Products run on fs (gemm2 when on2).
+struct Fold {
+  Ctx* c = nullptr;
+  hipStream_t fs = nullptr;
+  bool on2 = false;
+  int N = 0, n = 0, G = 0;
+  zc *P12, *P22, *P12n, *P22n, *Fm, *Gm, *Hm, *Id, *Mf, *vu, *X11, *X21, *va, *vb;
+  void* gj = nullptr;
+  int* info = nullptr;       // singular-pivot flags
+  const int* colidx = nullptr;  // device {N/2, N/2 + N}
+  int gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch);
+  // P = S_ref for the first cnt products, from their end diagonals (32 N
+  // complex per product, estride apart: S_ref's four blocks then S_trn's)
+  rcwa_status start(const zc* ends, long long estride, int cnt);
+  // P <- P * X for the first na of the G products (the others keep P)
+  rcwa_status step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool columns_only);
+  // P <- P * S_trn for the first cnt products, the columns only
+  rcwa_status finish(const zc* ends, long long estride, int cnt);
+};
+
+// Argument checks of one problem (devices, sources, media): the server
+// applies them at submission, rcwa_solve* per call.
+rcwa_status validate_problem(const rcwa_problem& p);
+
 // A problem with the place its results go.
 struct ProblemRef {
   rcwa_problem p;
@@ -118,3 +148,7 @@ struct SweepPlan {
 rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_source* src,
                         int nsrc, const rcwa_c128* eps, const rcwa_c128* mu,
                         rcwa_precision prec, rcwa_results* out);
+
+// k_z = 0 in a half-space, or in the gap medium when has_layers: RCWA_E_SINGULAR.
+rcwa_status check_wave_vectors(const rcwa_host::WaveVectors& wv, bool has_layers, int hx,
+                               int hy);

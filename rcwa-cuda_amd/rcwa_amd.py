@@ -182,7 +182,7 @@ class TicketInfo(ctypes.Structure):
 # Every function declared in include/rcwa_amd.h.
 EXPORTS = [
     "rcwa_last_error", "rcwa_version", "rcwa_ctx_create", "rcwa_ctx_destroy",
-    "rcwa_ctx_set_eig_maxit", "rcwa_ctx_set_qr_log", "rcwa_ctx_get_stats", "rcwa_ctx_reset_stats", "rcwa_ctx_set_chunk",
+    "rcwa_ctx_set_eig_maxit", "rcwa_ctx_set_qr_log", "rcwa_ctx_set_refine", "rcwa_ctx_get_stats", "rcwa_ctx_reset_stats", "rcwa_ctx_set_chunk",
     "rcwa_ctx_set_timing",
     "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
@@ -286,6 +286,9 @@ class Context:
 
     def set_qr_log(self, rotations_per_matrix):
         _check(lib().rcwa_ctx_set_qr_log(self.h, ctypes.c_longlong(rotations_per_matrix)))
+
+    def set_refine(self, level):
+        _check(lib().rcwa_ctx_set_refine(self.h, level))
 
     def set_timing(self, on):
         _check(lib().rcwa_ctx_set_timing(self.h, 1 if on else 0))
