@@ -478,7 +478,18 @@ int Fold::gemm(int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D
              : c->gemm(M_, N_, K_, al, A_, B_, be, D_, C_, bt);
 }
 
+// Every work slot, the column indices and the flags must be set before a
+// fold call: the kernels index through them without further checks.
+static rcwa_status fold_check(const Fold& f) {
+  const zc* p[] = {f.P12, f.P22, f.P12n, f.P22n, f.Fm, f.Gm, f.Hm, f.Id, f.Mf, f.vu, f.X11, f.X21, f.va, f.vb};
+  for (const zc* q : p)
+    if (!q) return rcwa_set_error(RCWA_E_ARG, "fold work slot not set");
+  if (!f.colidx || !f.gj || !f.c || !f.fs) return rcwa_set_error(RCWA_E_ARG, "fold not set up");
+  return RCWA_OK;
+}
+
 rcwa_status Fold::start(const zc* ends, long long estride, int cnt) {
+  if (rcwa_status st = fold_check(*this)) return st;
   const long long n2 = (long long)n * n;
   const zc one = zmake(1, 0);
   auto Mt = [&](zc* p) { return ZMat{p, n, n2}; };
@@ -495,6 +506,9 @@ rcwa_status Fold::start(const zc* ends, long long estride, int cnt) {
 }
 
 rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool columns_only) {
+  if (rcwa_status st = fold_check(*this)) return st;
+  if (!info) return rcwa_set_error(RCWA_E_ARG, "fold without inverse flags");
+  if (na > G) return rcwa_set_error(RCWA_E_ARG, "fold step over more products than it holds");
   const long long n2 = (long long)n * n;
   const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
   const ZCMat none{nullptr, 0, 0};

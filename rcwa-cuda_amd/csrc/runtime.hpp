@@ -102,7 +102,9 @@ struct Fold {
   hipStream_t fs = nullptr;
   bool on2 = false;
   int N = 0, n = 0, G = 0;
-  zc *P12, *P22, *P12n, *P22n, *Fm, *Gm, *Hm, *Id, *Mf, *vu, *X11, *X21, *va, *vb;
+  zc *P12 = nullptr, *P22 = nullptr, *P12n = nullptr, *P22n = nullptr, *Fm = nullptr,
+     *Gm = nullptr, *Hm = nullptr, *Id = nullptr, *Mf = nullptr, *vu = nullptr, *X11 = nullptr,
+     *X21 = nullptr, *va = nullptr, *vb = nullptr;
   void* gj = nullptr;
   int* info = nullptr;       // singular-pivot flags
   const int* colidx = nullptr;  // device {N/2, N/2 + N}

@@ -9,14 +9,13 @@ and 2 to 29 layers), submitted together to one server.
     order to 1e-9;
   - the stream made lossless (every permittivity real) conserves energy,
     R + T = 1 to 1e-10, for every request, the 25x25-harmonic 29-layer ones
-    included, where the oracle would take hours -- on requests whose layers
-    are well conditioned.  The bound scales with the conditioning the solve
-    reports (rcwa_results.cond, sqrt(n) ||W^-1||_F of the worst layer) above
-    1e3: request 13 (21x21 harmonics, 6 layers, cond 5.9e3) has layer
-    S-matrices of entries up to 1.6e2 whose (I - ...)^-1 has cond 1.8e4, and
-    the reference's own formulation on LAPACK moves them by 3e-10 to 6e-10
-    when Omega^2 is perturbed by one part in 1e15, the GPU result's distance
-    from it; R + T - 1 there is 3.3e-10."""
+    included, where the oracle would take hours.  The conditioning the solve
+    reports (rcwa_results.cond, sqrt(n) ||W^-1||_F of the worst layer) reaches
+    2.8e4 on this stream; the fold's solves carry one step of iterative
+    refinement, without which the worst request (13: 21x21 harmonics, 6
+    layers, layer S-matrix entries up to 1.6e2 whose (I - ...)^-1 has
+    condition number 1.8e4) drifts to R + T - 1 = 3.3e-10.  cond is finite
+    and at least sqrt(n) >= 1 (W has unit columns, so ||W||_2 <= sqrt(n))."""
 import numpy as np
 import pytest
 
@@ -68,8 +67,8 @@ def test_c4_lossless_energy(stream):
     served = _serve(lossless)
     for p, (R, T, tot, info) in zip(lossless, served):
         dev = np.abs(tot.sum(axis=1) - 1.0).max()
-        bound = 1e-10 * max(1.0, info["cond"] / 1e3)
-        print("h=%d L=%d: R + T - 1 = %.2e (cond %.2e, bound %.1e)"
-              % (p.hx, p.L, dev, info["cond"], bound))
-        assert dev < bound
+        print("h=%d L=%d: R + T - 1 = %.2e (cond %.2e)" % (p.hx, p.L, dev, info["cond"]))
+        assert dev < 1e-10
+        if p.L > 0:
+            assert np.isfinite(info["cond"]) and info["cond"] >= 1.0
         assert R.min() > -1e-12 and T.min() > -1e-12
