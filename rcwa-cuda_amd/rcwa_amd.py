@@ -588,13 +588,18 @@ class Server:
         _check(lib().rcwa_server_create(device, ctypes.byref(cfg), ctypes.byref(self.h)))
         self._meta = {}
 
-    def submit(self, p):
+    def submit(self, p, sources=None):
+        """Queues problem p; its sources are p's polarisations at p's
+        incidence, or `sources`: (lam0, theta, phi, pte, ptm) tuples, which
+        may name several incidences (each runs as its own flight)."""
         er, ur, thick = _grids(p)
         L, Ny, Nx = er.shape
         d = Device(Nx, Ny, L, p.Lx, p.Ly, er.ctypes.data, ur.ctypes.data, thick.ctypes.data, 0)
         par = Params(p.hx, p.hy, _c(p.er_ref), _c(p.ur_ref), _c(p.er_trn), _c(p.ur_trn))
-        npol = len(p.pols)
-        srcs = (Source * npol)(*[Source(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols])
+        if sources is None:
+            sources = [(p.lam0, p.theta, p.phi, a, b) for a, b in p.pols]
+        npol = len(sources)
+        srcs = (Source * npol)(*[Source(*s_) for s_ in sources])
         prob = Problem(ctypes.pointer(d), ctypes.pointer(par), srcs, npol)
         t = ctypes.c_uint64()
         _check(lib().rcwa_server_submit(self.h, ctypes.byref(prob), ctypes.byref(t)))

@@ -97,3 +97,29 @@ def test_cancel_wakes_a_waiter_and_tickets_are_collected_once():
             srv.poll(ts[0])                      # already collected
         for t in ts[1:-1]:
             srv.wait(t)
+
+
+@pytest.mark.parametrize("max_batch", [1, 0])
+def test_request_of_several_incidences(ctx, max_batch):
+    """A request whose sources name three incidences (three wavelengths, one
+    at another angle) runs as three flights -- in parts, one after another,
+    on a server of one slot (max_batch=1), side by side on the default one --
+    and each source's result is the one the single-incidence solve gives."""
+    import copy
+    rng = np.random.default_rng(31)
+    p = W.random_small(rng, hx=2, hy=2, L=3, grid=16)
+    incid = [(p.lam0, p.theta, p.phi), (1.3 * p.lam0, p.theta, p.phi),
+             (0.8 * p.lam0, 0.5 * p.theta, p.phi + 0.3)]
+    sources = [(lam, th, ph, a, b) for lam, th, ph in incid for a, b in p.pols]
+    with G.Server(0, hbm_budget=4 << 30, n_workers=1, max_batch=max_batch) as srv:
+        t = srv.submit(p, sources=sources)
+        R, T, tot, info = srv.wait(t)
+    npol = len(p.pols)
+    for k, (lam, th, ph) in enumerate(incid):
+        q = copy.copy(p)
+        q.lam0, q.theta, q.phi = lam, th, ph
+        Ra, Ta, tota = ctx.solve([q])[0]
+        sl = slice(k * npol, (k + 1) * npol)
+        assert np.abs(R[sl] - Ra).max() < 1e-10
+        assert np.abs(T[sl] - Ta).max() < 1e-10
+        assert np.abs(tot[sl] - tota).max() < 1e-10
