@@ -9,12 +9,15 @@ The JSON line reports:
   value        solves/s (one solve = one wavelength with both polarisations)
   latency_ms   p50/p99 of the per-step wall time (every solve of a step
                completes with the step)
-  roofline     the QR phase of the eigensolver (sweep kernel, log apply to
-               the Schur vectors, the two products forming the Schur form;
-               the dominant phase), whose bound is HBM traffic: its
-               compulsory bytes per launch over its event-timed average
-               duration, against the HBM peak, beside the traffic measured
-               with PMC counters and its FP64 flop rate
+  roofline     the QR phase of the eigensolver (multishift sweep kernel with
+               aggressive early deflation, log apply to the Schur vectors,
+               the two products forming the Schur form; the dominant phase),
+               classed as HBM-bound (no matrix-core work): its compulsory
+               bytes per launch over its event-timed average duration,
+               against the HBM peak, beside the traffic measured with PMC
+               counters (about 99x the compulsory bytes, moved at 2.7 TB/s:
+               the phase is bound by the latency of its one-wave steps, not
+               by HBM) and its FP64 flop rate
   split_sample the split-precision form on 8 of the timed wavelengths:
                max |dR|, |dT| against the FP64 results of this run
   cpu_baseline the CPU FP64 oracle on a bounded sample of the same
@@ -46,9 +49,10 @@ sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
 METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
 # HBM bytes (read + written) of the QR phase per 882x882 layer matrix, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR phase's kernels at
-# 510 matrices per launch, scaled by the counters' reading on a copy of known
-# size (--hbm-copy-probe; profiles/qr_phase_21x21_pmc_traffic.md).
-QR_PHASE_HBM_BYTES_PER_MATRIX = 6727.9e9 / 510
+# 510 matrices per launch (the mean of the step's two launches, 3151.1 and
+# 3111.0 GB), scaled by the counters' reading on a copy of known size
+# (--hbm-copy-probe; profiles/qr_phase_21x21_pmc_traffic.md).
+QR_PHASE_HBM_BYTES_PER_MATRIX = 3131.05e9 / 510
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
 
@@ -255,6 +259,9 @@ def main():
     ap.add_argument("--split", action="store_true", help="also time the split-precision form")
     ap.add_argument("--planner", action="store_true", help="also time the sweep-plan form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split-sample", action="store_true",
+                    help="leave out the split-precision accuracy sample (outside the timed "
+                         "region), e.g. for counter passes over the FP64 path alone")
     ap.add_argument("--refine", type=int, default=None,
                     help="fold refinement level (rcwa_ctx_set_refine; default: the library's)")
     ap.add_argument("--c5", action="store_true",
@@ -358,10 +365,11 @@ def main():
     compulsory = 5.0 * n * n * 16
     gbps = compulsory * per_launch_mats / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats
-    roofline = {"kernel": "QR phase: schur_pipe_kernel + qr_apply_kernel + 2 x zgemm_kernel "
-                          "(complex single-shift QR sweeps on the Hessenberg matrix, two "
-                          "chained per pass; their rotations applied to the Schur vectors Z; "
-                          "Schur form T = Z^H A Z)",
+    roofline = {"kernel": "QR phase: schur_ms_kernel + qr_apply_ms_kernel + 2 x zgemm_kernel "
+                          "(complex multishift QR with aggressive early deflation on the "
+                          "Hessenberg matrix: 32x32 deflation windows, up to eight chained "
+                          "single-shift sweeps per pass; the logged rotations and window "
+                          "blocks applied to the Schur vectors Z; Schur form T = Z^H A Z)",
                 "bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": gbps / HBM_PEAK_GBPS, "traffic": traffic,
                 "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE passes, "
@@ -395,17 +403,20 @@ def main():
     # the split-precision form on 8 of the timed wavelengths, against this run's
     # FP64 results (the metric's max |dR| vs FP64)
     flat = [r for step in results for r in step]
-    pick = np.linspace(0, len(flat) - 1, min(8, len(flat))).round().astype(int).tolist()
-    sel = [probs[args.warmup * args.lams + i] for i in pick]
-    grids = [(er_devs[args.warmup * args.lams + i], ur_dev) for i in pick]
-    sout = ctx.solve(sel, G.RCWA_SPLIT, device_grids=grids, grid_shape=(512, 512))
-    out["split_sample"] = {
-        "wavelengths": len(pick),
-        "max_dR_vs_fp64": float(max(np.abs(flat[i][0] - o[0]).max() for i, o in zip(pick, sout))),
-        "max_dT_vs_fp64": float(max(np.abs(flat[i][1] - o[1]).max() for i, o in zip(pick, sout))),
-        "max_dtot_vs_fp64": float(max(np.abs(flat[i][2] - o[2]).max()
-                                      for i, o in zip(pick, sout)))}
-    progress("split-precision sample done")
+    if not args.no_split_sample:
+        pick = np.linspace(0, len(flat) - 1, min(8, len(flat))).round().astype(int).tolist()
+        sel = [probs[args.warmup * args.lams + i] for i in pick]
+        grids = [(er_devs[args.warmup * args.lams + i], ur_dev) for i in pick]
+        sout = ctx.solve(sel, G.RCWA_SPLIT, device_grids=grids, grid_shape=(512, 512))
+        out["split_sample"] = {
+            "wavelengths": len(pick),
+            "max_dR_vs_fp64": float(max(np.abs(flat[i][0] - o[0]).max()
+                                        for i, o in zip(pick, sout))),
+            "max_dT_vs_fp64": float(max(np.abs(flat[i][1] - o[1]).max()
+                                        for i, o in zip(pick, sout))),
+            "max_dtot_vs_fp64": float(max(np.abs(flat[i][2] - o[2]).max()
+                                          for i, o in zip(pick, sout)))}
+        progress("split-precision sample done")
 
     if args.split:
         sres = []
