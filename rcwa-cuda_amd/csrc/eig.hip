@@ -938,6 +938,17 @@ __global__ __launch_bounds__(NT) void schur_defer_kernel(int n, ZMat Tm, ZMat Zm
   }
 }
 
+// The lane index through an opaque move.  Index arithmetic derived from it
+// is then recomputed inside the loop that uses it; derived from the plain lane
+// index it is loop-invariant, and the compiler hoists it out of every loop
+// and holds it in registers for the whole kernel (about 150 registers in the
+// strips of the multishift kernel alone).
+__device__ __forceinline__ int lane_here() {
+  int l;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(l) : "v"((int)(threadIdx.x & 63)));
+  return l;
+}
+
 // Raw buffer access to one matrix: offsets at or past the end read zero and
 // drop stores, so masked-off lanes need no branch (and the loads of a batch
 // stay in flight together).
@@ -1006,6 +1017,53 @@ __device__ __forceinline__ void rotate_window_staged(zc* x, bool colform, const 
       const zc a = x[o], b = x[o + 1];
       x[o] = zadd(zscale(a, cs), zmul(s, b));
       x[o + 1] = zsub(zscale(b, cs), zmul(zconj(s), a));
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// rotate_window_staged with the rotations of step t + 1 read from LDS while
+// step t computes (the strips of the multishift kernel, whose 64 lanes apply
+// every rotation: the LDS latency of a step's NC rotations was exposed once
+// per step).
+template <int R, int NC, int GAP>
+__device__ __forceinline__ void rotate_window_pipe(zc* x, bool colform, const double* rc,
+                                                   const zc* rs) {
+  double cc[NC];
+  zc ss[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    cc[c] = rc[c];
+    ss[c] = rs[c];
+  }
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    __builtin_amdgcn_sched_barrier(0);
+    double cn[NC];
+    zc sn[NC];
+    if (t + 1 < R) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        cn[c] = rc[NC * (t + 1) + c];
+        sn[c] = rs[NC * (t + 1) + c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int o = GAP * (NC - 1 - c) + t;
+      const double cs = cc[c];
+      zc s = ss[c];
+      if (colform) s.y = -s.y;
+      const zc a = x[o], b = x[o + 1];
+      x[o] = zadd(zscale(a, cs), zmul(s, b));
+      x[o + 1] = zsub(zscale(b, cs), zmul(zconj(s), a));
+    }
+    if (t + 1 < R) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        cc[c] = cn[c];
+        ss[c] = sn[c];
+      }
     }
   }
   __builtin_amdgcn_sched_barrier(0);
@@ -1437,15 +1495,53 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// A double of lane l (uniform) in every lane.
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ zc readlane_z(zc v, int l) {
+  return zmake(readlane_f64(v.x, l), readlane_f64(v.y, l));
+}
+
+// One lane's share of a rotation in the AED window: the pair (x, y) =
+// (p[0], p[stride]) becomes (c x + s1 y, c y - conj(s1) x) -- a row rotation
+// (rows q, q+1 of one column: stride 1, s1 = s) or a column rotation (columns
+// q, q+1 of one row: stride LDW, s1 = conj(s)).  Every lane runs the same
+// instructions; lanes without a pair (act false) read a valid address and do
+// not store.  Returns the new x.
+__device__ __forceinline__ zc lane_rotate(zc* p, int stride, bool rowform, bool act, double c,
+                                          zc s) {
+  const zc x = p[0], y = p[stride];
+  const zc s1 = zmake(s.x, rowform ? s.y : -s.y);
+  const zc xn = zadd(zscale(x, c), zmul(s1, y));
+  const zc yn = zsub(zscale(y, c), zmul(zconj(s1), x));
+  if (act) {
+    p[0] = xn;
+    p[stride] = yn;
+  }
+  return xn;
+}
+
 // Complex Schur form of the nw x nw upper Hessenberg Tw (leading dimension
-// NW + 1, nw <= 64) by single-shift QR sweeps, each rotation applied to the whole
-// window and accumulated into Vw (starts as the identity).  One wave; every
-// lane follows the same control flow from LDS values.  Wilkinson shifts with
-// the exceptional shifts of schur_pipe_kernel at the 10th and 20th sweep on
-// one block.  Returns false when the sweep budget runs out; *sweeps counts
-// the sweeps done.
+// NW + 1, nw <= NW <= 32) by single-shift QR sweeps, each rotation applied to
+// the whole window and accumulated into Vw (starts as the identity).  One
+// wave; every lane follows the same control flow.  Wilkinson shifts with the
+// exceptional shifts of schur_pipe_kernel at the 10th and 20th sweep on one
+// block.  Returns false when the sweep budget runs out; *sweeps counts the
+// sweeps done.
+//
+// Rotation q of a sweep: rows q, q+1 of columns q .. nw-1 (lane l: column
+// q+l) together with columns q, q+1 of Vw (lane 32+i: row i), then columns
+// q, q+1 of rows 0 .. min(q+2, hi) of Tw (lane i: row i).  The next
+// rotation's inputs T(q+1, q), T(q+2, q) are the results of lanes q+1, q+2
+// of the second pass, taken by readlane.
 template <int NW>
 __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, int& sweeps) {
+  static_assert(NW <= 32, "one lane per window row in each half-wave");
   constexpr int LDW = NW + 1;
   int hi = nw - 1, it = 0;
   const double eps = DBL_EPSILON;
@@ -1480,15 +1576,10 @@ __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, 
     else
       mu = wilkinson(Tw[(hi - 1) + (hi - 1) * LDW], Tw[(hi - 1) + hi * LDW], Tw[hi + (hi - 1) * LDW],
                      Tw[hi + hi * LDW]);
+    zc f = zsub(Tw[lo + lo * LDW], mu), g = Tw[(lo + 1) + lo * LDW];
+    const bool vlane = lane >= 32;
+    const int vi = lane - 32;
     for (int q = lo; q < hi; ++q) {
-      zc f, g;
-      if (q == lo) {
-        f = zsub(Tw[lo + lo * LDW], mu);
-        g = Tw[(lo + 1) + lo * LDW];
-      } else {
-        f = Tw[q + (q - 1) * LDW];
-        g = Tw[(q + 1) + (q - 1) * LDW];
-      }
       double c;
       zc s, r;
       givens_rsq(f, g, c, s, r);
@@ -1496,72 +1587,19 @@ __device__ bool small_schur(zc* Tw, zc* Vw, int nw, int lane, long long budget, 
         Tw[q + (q - 1) * LDW] = r;
         Tw[(q + 1) + (q - 1) * LDW] = zmake(0, 0);
       }
-      const zc sc = zconj(s);
-      const int j = q + lane;  // row rotation: columns q .. nw-1
-      if (j < nw) {
-        zc* p0 = &Tw[q + j * LDW];
-        const zc x = p0[0], y = p0[1];
-        p0[0] = zadd(zscale(x, c), zmul(s, y));
-        p0[1] = zsub(zscale(y, c), zmul(sc, x));
-      }
-      wave_sync();
-      const int i = lane;  // column rotation: rows 0 .. min(q + 2, hi), and Vw
-      if (i <= min(q + 2, hi)) {
-        zc* q0 = &Tw[i + q * LDW];
-        const zc x = q0[0], y = q0[LDW];
-        q0[0] = zadd(zscale(x, c), zmul(sc, y));
-        q0[LDW] = zsub(zscale(y, c), zmul(s, x));
-      }
-      if (i < nw) {
-        zc* v0 = &Vw[i + q * LDW];
-        const zc x = v0[0], y = v0[LDW];
-        v0[0] = zadd(zscale(x, c), zmul(sc, y));
-        v0[LDW] = zsub(zscale(y, c), zmul(s, x));
-      }
-      wave_sync();
+      // rows q, q+1 of column q + lane; columns q, q+1 of row vi of Vw
+      const bool a1 = vlane ? vi < nw : q + lane < nw;
+      zc* const p1 = vlane ? (a1 ? &Vw[vi + q * LDW] : Vw) : (a1 ? &Tw[q + (q + lane) * LDW] : Tw);
+      lane_rotate(p1, vlane ? LDW : 1, !vlane, a1, c, s);
+      // columns q, q+1 of row `lane` (rows 0 .. min(q + 2, hi))
+      const bool a2 = !vlane && lane <= min(q + 2, hi);
+      const zc xn = lane_rotate(a2 ? &Tw[lane + q * LDW] : Tw, LDW, false, a2, c, s);
+      f = readlane_z(xn, q + 1);
+      g = readlane_z(xn, min(q + 2, 31));
     }
+    wave_sync();
   }
   return true;
-}
-
-// Swaps the adjacent diagonal entries q, q+1 of the upper triangular Tw
-// (LAPACK's ztrexc step): a rotation from (T(q, q+1), T(q+1, q+1) - T(q, q))
-// on rows q, q+1 right of the pair, on columns q, q+1 above it and on Vw; the
-// diagonal entries trade places and T(q, q+1) keeps its value.
-template <int NW>
-__device__ void swap_adjacent(zc* Tw, zc* Vw, int nw, int q, int lane) {
-  constexpr int LDW = NW + 1;
-  const zc t11 = Tw[q + q * LDW], t22 = Tw[(q + 1) + (q + 1) * LDW];
-  double c;
-  zc s, r;
-  givens_rsq(Tw[q + (q + 1) * LDW], zsub(t22, t11), c, s, r);
-  const zc sc = zconj(s);
-  const int j = q + 2 + lane;
-  if (j < nw) {
-    zc* p0 = &Tw[q + j * LDW];
-    const zc x = p0[0], y = p0[1];
-    p0[0] = zadd(zscale(x, c), zmul(s, y));
-    p0[1] = zsub(zscale(y, c), zmul(sc, x));
-  }
-  const int i = lane;
-  if (i < q) {
-    zc* q0 = &Tw[i + q * LDW];
-    const zc x = q0[0], y = q0[LDW];
-    q0[0] = zadd(zscale(x, c), zmul(sc, y));
-    q0[LDW] = zsub(zscale(y, c), zmul(s, x));
-  }
-  if (i < nw) {
-    zc* v0 = &Vw[i + q * LDW];
-    const zc x = v0[0], y = v0[LDW];
-    v0[0] = zadd(zscale(x, c), zmul(sc, y));
-    v0[LDW] = zsub(zscale(y, c), zmul(s, x));
-  }
-  if (lane == 0) {
-    Tw[q + q * LDW] = t22;
-    Tw[(q + 1) + (q + 1) * LDW] = t11;
-    Tw[(q + 1) + q * LDW] = zmake(0, 0);
-  }
-  wave_sync();
 }
 
 // The deflation test of the AED window (zlaqr3): from the bottom, an
@@ -1569,20 +1607,54 @@ __device__ void swap_adjacent(zc* Tw, zc* Vw, int nw, int q, int lane) {
 // size deflates; one that is not is moved up to the top of the undeflated
 // entries.  Returns ns, the undeflated count: they occupy 0 .. ns-1, the
 // deflated ones ns .. nw-1.
+//
+// Moving the eigenvalue t at j up to ilst is a chain of swaps of adjacent
+// diagonal entries (LAPACK's ztrexc step): swap q takes the rotation of
+// (T(q, q+1), t - T(q, q)) to rows q, q+1 right of the pair, to columns q,
+// q+1 above it and to Vw; the diagonal entries trade places and T(q, q+1)
+// keeps its value.  One rotate per lane per swap, as in small_schur (lane
+// l < nw-q-2: column q+2+l; lanes nw-q-2 .. nw-3: rows 0 .. q-1; lanes 32
+// ..: the rows of Vw; lane nw-1 stores the diagonal); the next swap's
+// T(q-1, q) is formed in every lane from T(q-1, q:q+1), read one swap ahead.
 template <int NW>
 __device__ int aed_deflate(zc* Tw, zc* Vw, int nw, zc spike, double smlnum, int lane) {
+  static_assert(NW <= 32, "one lane per window row in each half-wave");
   constexpr int LDW = NW + 1;
   const double s1 = zabs1(spike);
   int ns = nw, ilst = 0;
   while (ns > ilst) {
     const int j = ns - 1;
-    double foo = zabs1(Tw[j + j * LDW]);
+    const zc tm = Tw[j + j * LDW];
+    double foo = zabs1(tm);
     if (foo == 0.0) foo = s1;
     if (s1 * zabs1(Vw[j * LDW]) <= fmax(smlnum, DBL_EPSILON * foo)) {
       --ns;
       continue;
     }
-    for (int q = j - 1; q >= ilst; --q) swap_adjacent<NW>(Tw, Vw, nw, q, lane);
+    zc tq = Tw[(j - 1) + j * LDW];
+    for (int q = j - 1; q >= ilst; --q) {
+      const zc t11 = Tw[q + q * LDW];
+      const int qm = q > 0 ? q - 1 : 0;
+      const zc u0 = Tw[qm + q * LDW], u1 = Tw[qm + (q + 1) * LDW];
+      const int nA = nw - q - 2;
+      const bool rowform = lane < nA;
+      const bool act = rowform || (lane >= nA && lane < nw - 2) || (lane >= 32 && lane - 32 < nw);
+      const int off = rowform ? q + (q + 2 + lane) * LDW
+                      : lane < 32 ? (act ? (lane - nA) + q * LDW : 0)
+                                  : (act ? (lane - 32) + q * LDW : 0);
+      zc* const p = (lane < 32 ? Tw : Vw) + off;
+      double c;
+      zc s, r;
+      givens_rsq(tq, zsub(tm, t11), c, s, r);
+      lane_rotate(p, rowform ? 1 : LDW, rowform, act, c, s);
+      if (lane == nw - 1) {
+        Tw[q + q * LDW] = tm;
+        Tw[(q + 1) + (q + 1) * LDW] = t11;
+        Tw[(q + 1) + q * LDW] = zmake(0, 0);
+      }
+      tq = zadd(zscale(u0, c), zmul(zconj(s), u1));
+    }
+    wave_sync();
     ++ilst;
   }
   return ns;
@@ -1612,33 +1684,49 @@ __device__ zc house(const zc* x, int len, zc* hv, zc* beta, int lane) {
   return tau;
 }
 
-// Rows r0 .. r0+len-1 of Tw, columns c0 .. nw-1: Tw <- H^H Tw.
+// Rows r0 .. r0+len-1 of Tw, columns c0 .. nw-1: Tw <- H^H Tw.  The loops
+// run over the static bound NW (terms past len are masked to zero) so that
+// their LDS reads issue together; every address stays inside the window's
+// LDS area.
 template <int NW>
 __device__ void house_left(zc* Tw, int nw, int r0, int len, int c0, const zc* hv, zc tau,
                            int lane) {
   constexpr int LDW = NW + 1;
   const int j = c0 + lane;
-  if (j < nw) {
+  if (lane < NW && j < nw) {
     zc w = zmake(0, 0);
-    for (int i = 0; i < len; ++i) w = zadd(w, zmul(zconj(hv[i]), Tw[(r0 + i) + j * LDW]));
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      const zc h = hv[i], t = Tw[(r0 + i) + j * LDW];
+      if (i < len) w = zadd(w, zmul(zconj(h), t));
+    }
     w = zmul(w, zconj(tau));
-    for (int i = 0; i < len; ++i)
-      Tw[(r0 + i) + j * LDW] = zsub(Tw[(r0 + i) + j * LDW], zmul(hv[i], w));
+#pragma unroll
+    for (int i = 0; i < NW; ++i)
+      if (i < len) Tw[(r0 + i) + j * LDW] = zsub(Tw[(r0 + i) + j * LDW], zmul(hv[i], w));
   }
   wave_sync();
 }
 
-// Columns c0 .. c0+len-1 of M (rows 0 .. nw-1): M <- M H.
+// Columns c0 .. c0+len-1 of Tw (lanes 0 .. nw-1, its rows) and of Vw (lanes
+// 32 .. 32+nw-1): M <- M H.  Static bounds as in house_left.
 template <int NW>
-__device__ void house_right(zc* M, int nw, int c0, int len, const zc* hv, zc tau, int lane) {
+__device__ void house_right2(zc* Tw, zc* Vw, int nw, int c0, int len, const zc* hv, zc tau,
+                             int lane) {
   constexpr int LDW = NW + 1;
-  const int i = lane;
+  const int i = lane & 31;
+  zc* M = lane < 32 ? Tw : Vw;
   if (i < nw) {
     zc w = zmake(0, 0);
-    for (int j = 0; j < len; ++j) w = zadd(w, zmul(M[i + (c0 + j) * LDW], hv[j]));
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      const zc h = hv[j], m = M[i + (c0 + j) * LDW];
+      if (j < len) w = zadd(w, zmul(m, h));
+    }
     w = zmul(w, tau);
-    for (int j = 0; j < len; ++j)
-      M[i + (c0 + j) * LDW] = zsub(M[i + (c0 + j) * LDW], zmul(w, zconj(hv[j])));
+#pragma unroll
+    for (int j = 0; j < NW; ++j)
+      if (j < len) M[i + (c0 + j) * LDW] = zsub(M[i + (c0 + j) * LDW], zmul(w, zconj(hv[j])));
   }
   wave_sync();
 }
@@ -1654,8 +1742,7 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
   zc beta;
   zc tau = house(sv, ns, hv, &beta, lane);
   house_left<NW>(Tw, nw, 0, ns, 0, hv, tau, lane);
-  house_right<NW>(Tw, nw, 0, ns, hv, tau, lane);
-  house_right<NW>(Vw, nw, 0, ns, hv, tau, lane);
+  house_right2<NW>(Tw, Vw, nw, 0, ns, hv, tau, lane);
   const zc spike = beta;
   for (int k = 0; k + 2 < ns; ++k) {
     const int len = ns - k - 1;
@@ -1666,8 +1753,7 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
     if (lane < len) Tw[(k + 1 + lane) + k * LDW] = lane == 0 ? bk : zmake(0, 0);
     wave_sync();
     house_left<NW>(Tw, nw, k + 1, len, k + 1, hv, tau, lane);
-    house_right<NW>(Tw, nw, k + 1, len, hv, tau, lane);
-    house_right<NW>(Vw, nw, k + 1, len, hv, tau, lane);
+    house_right2<NW>(Tw, Vw, nw, k + 1, len, hv, tau, lane);
   }
   return spike;
 }
@@ -1914,6 +2000,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
       zc* cb = band[w & 1];
       const zc* pbnd = band[(w + 1) & 1];
       if (wave == 0) {
+        const int lane = lane_here();
         // loads for window w+1: J(w+1), rows k0 - R - GP + q
         const int ce1 = wce(w + 1);
         const int nj1 = cur ? ce1 - ce : 0;
@@ -2090,6 +2177,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
         const int s0 = pk0 - GP;
         zc* tile = stile + sw * (64 * RS);
         for (int ch = sw; ch < nch2; ch += SW) {
+          const int lane = lane_here();
           zc v[RS];
           unsigned off[RS];
           const bool right = ch < nrc;
@@ -2118,7 +2206,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
 #pragma unroll
             for (int q = 0; q < RS; ++q) v[q] = tile[lane * RS + q];
           }
-          rotate_window_staged<R, NC, GAP>(v, !right, prc, prs);
+          rotate_window_pipe<R, NC, GAP>(v, !right, prc, prs);
           if (right) {
 #pragma unroll
             for (int q = 0; q < RS; ++q) tile[lane * RS + q] = v[q];
