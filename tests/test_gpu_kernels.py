@@ -215,11 +215,14 @@ def test_convmat_vs_oracle(ctx, grid, hx, hy):
         assert np.abs(c - ref).max() < 1e-14 * np.abs(g).max()
 
 
-@pytest.mark.parametrize("m,n,k", [(16, 16, 16), (70, 33, 100), (242, 242, 242)])
+@pytest.mark.parametrize("m,n,k", [(16, 16, 16), (70, 33, 100), (242, 242, 242),
+                                   (16, 40, 40000)])
 def test_zgemm_split_accuracy(ctx, m, n, k):
-    """Ozaki scheme with four 9-bit fixed-point FP16 slices and exact FP32
-    accumulation of the two leading groups of slice products: normwise error
-    about 2^-36 (the dropped slice products), independent of k."""
+    """Ozaki scheme with six 7-bit fixed-point int8 slices, the slice products
+    of weight 2^-49 and above accumulated exactly in int32 by weight group:
+    error below about 2^-37 of max|a_row| max|b_col| per k term (the dropped
+    slice products).  k = 40000 crosses the 2^15-k chunks the int32 sums are
+    cut into."""
     rng = np.random.default_rng(m + n + k)
     A = [rnd(rng, m, k) * np.exp(rng.uniform(-3, 3, (m, 1))) for _ in range(2)]
     B = [rnd(rng, k, n) for _ in range(2)]
