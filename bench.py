@@ -172,7 +172,15 @@ def serve_bench(args):
     import rcwa_amd as G
     import workloads as W
     stream = W.c4_stream(count=args.serve)
-    with G.Server(0, n_workers=2) as srv:
+    if args.autotune:
+        # warm-up pass: the shape classes the stream meets are tuned here,
+        # outside the measured pass (the tuned table is process-wide)
+        with G.Server(0, n_workers=2, autotune=True) as srv:
+            for t in [srv.submit(p) for _, p in stream]:
+                srv.wait(t)
+        print("serving warm-up pass done (%d tuned shape classes)" % len(G.autotune_entries()),
+              file=sys.stderr, flush=True)
+    with G.Server(0, n_workers=2, autotune=args.autotune) as srv:
         tickets = [None] * len(stream)
         t0 = time.perf_counter()
 
@@ -197,6 +205,7 @@ def serve_bench(args):
     served = len(res) / (last - first)
     nsample = len(stream)
     ctx = G.Context(0)
+    ctx.set_autotune(args.autotune)
     t1 = time.perf_counter()
     alone = []
     for i, (_, p) in enumerate(stream[:nsample]):
@@ -212,6 +221,8 @@ def serve_bench(args):
             "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                            "p99": float(np.percentile(lat, 99) * 1e3)},
             "mean_batch": float(np.mean([r[3]["batch_size"] for r in res])),
+            "tuned_shape_classes": len(G.autotune_entries()),
+            "tuned_nondefault": sum(1 for e in G.autotune_entries() if e[5] != 0),
             "max_dR_mix_vs_alone": dmax, "clock_start": c0}
 
 
@@ -266,6 +277,9 @@ def main():
                     help="fold refinement level (rcwa_ctx_set_refine; default: the library's)")
     ap.add_argument("--c5", action="store_true",
                     help="also time the 41x41-harmonic, 50-layer single structure")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the complex GEMM tile shapes on first use (rcwa_ctx_set_autotune; "
+                         "the warm-up step pays the tuning)")
     ap.add_argument("--serve", type=int, default=0,
                     help="also run the serving form on this many requests of the stream")
     ap.add_argument("--cpu-sample", choices=["layer", "star", "both"], help=argparse.SUPPRESS)
@@ -319,6 +333,7 @@ def main():
     hip.sync()
 
     ctx = G.Context(0)
+    ctx.set_autotune(args.autotune)
     if args.refine is not None:
         ctx.set_refine(args.refine)
 

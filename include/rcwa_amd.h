@@ -149,6 +149,22 @@ rcwa_status rcwa_ctx_set_chunk(rcwa_ctx* ctx, int max_chunk);
  * Stacks whose layer S-matrices have large entries make M ill-conditioned;
  * there the refined solves keep R + T = 1 for lossless stacks. */
 rcwa_status rcwa_ctx_set_refine(rcwa_ctx* ctx, int level);
+/* Tile-shape autotuning of the complex GEMMs (no counterpart in the
+ * reference, whose products are Eigen's): while on, the first product of a
+ * shape class (M, N, K, batch by its power of two) that a solve, sweep or
+ * server worker of this context meets is timed in every tile shape on
+ * scratch operands (a device allocation and a stream synchronisation, once
+ * per class and process) and the fastest is kept for the process.  Every
+ * tile shape gives the same bits, so results do not depend on the choice.
+ * RCWA_TUNE_FILE names a file the table is read from at first use and
+ * appended to.  Off by default. */
+rcwa_status rcwa_ctx_set_autotune(rcwa_ctx* ctx, int on);
+/* The tuned table: six ints per entry (M, N, K, batch class, device, tile
+ * shape) into out, up to max_entries; returns the number of entries. */
+int rcwa_autotune_entries(int* out, int max_entries);
+/* Forces one tile shape (0 .. 4) for every complex GEMM of the process, or
+ * -1 for the tuned ones: for tests and comparisons. */
+rcwa_status rcwa_zgemm_force_shape(int shape);
 /* Per-phase eigensolver timing with HIP events (adds a synchronisation). */
 rcwa_status rcwa_ctx_set_timing(rcwa_ctx* ctx, int on);
 
@@ -352,6 +368,7 @@ typedef struct {
   int max_batch;         /* largest batched launch (problems), 0 = auto */
   int n_workers;         /* concurrent GPU streams, 0 = 2 */
   rcwa_precision precision;
+  int autotune;          /* 1: ZGEMM tile shapes tuned on first use (rcwa_ctx_set_autotune) */
 } rcwa_server_config;
 
 typedef struct {

@@ -12,6 +12,23 @@ int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B
 // written); ZGEMM_TRI_B takes B as upper triangular (its entries below the
 // diagonal are not read); ZGEMM_TRI_A likewise takes A as upper triangular.
 constexpr int ZGEMM_TRI_C = 1, ZGEMM_TRI_B = 2, ZGEMM_TRI_A = 4;
+// Tile-shape autotuner of zgemm_batched*: while a ZgemmTuneScope(true) is
+// alive on a thread, a product of a shape class (M, N, K, batch by its power
+// of two, device) met for the first time is timed on scratch operands in
+// every tile shape (a hipMalloc and a synchronisation of the stream) and the
+// fastest is kept for the process; elsewhere known classes use their tuned
+// shape and unknown ones the default.  Every shape gives the same bits.
+struct ZgemmTuneScope {
+  explicit ZgemmTuneScope(bool on);
+  ~ZgemmTuneScope();
+  bool prev;
+};
+int zgemm_shape_count();
+// Forces one tile shape for every product (tests and comparisons); -1 undoes.
+void zgemm_force_shape(int shape);
+// The tuned table: 6 ints per entry (M, N, K, batch class, device, shape)
+// into out (up to max_entries); returns the number of entries.
+int zgemm_tuned_entries(int* out, int max_entries);
 int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
                       ZCMat D, ZMat C, int batch, int tri);
 

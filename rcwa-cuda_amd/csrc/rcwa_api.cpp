@@ -972,6 +972,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
 }
 
 rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_precision prec) {
+  ZgemmTuneScope tune(c.autotune);
   if (rcwa_status st = c.set_precision(prec)) return st;
   c.last_cols.clear();
   std::vector<Job> jobs;
@@ -1050,6 +1051,7 @@ rcwa_status solve_problems(Ctx& c, const ProblemRef* probs, int nprob, rcwa_prec
 rcwa_status solve_sweep(Ctx& c, const SweepPlan& plan, int npoints, const rcwa_source* src,
                         int nsrc, const rcwa_c128* eps, const rcwa_c128* mu,
                         rcwa_precision prec, rcwa_results* out) {
+  ZgemmTuneScope tune(c.autotune);
   if (npoints < 0 || (npoints > 0 && (!src || !eps || !out)) || nsrc <= 0)
     return rcwa_set_error(RCWA_E_ARG, "bad sweep arguments");
   if (rcwa_status st = c.set_precision(prec)) return st;
@@ -1181,6 +1183,22 @@ rcwa_status rcwa_ctx_set_qr_log(rcwa_ctx* ctx, long long rotations) {
 rcwa_status rcwa_ctx_set_refine(rcwa_ctx* ctx, int level) {
   if (!ctx || level < 0 || level > 2) return rcwa_set_error(RCWA_E_ARG, "bad argument");
   reinterpret_cast<Ctx*>(ctx)->refine = level;
+  return RCWA_OK;
+}
+
+rcwa_status rcwa_ctx_set_autotune(rcwa_ctx* ctx, int on) {
+  if (!ctx) return rcwa_set_error(RCWA_E_ARG, "null ctx");
+  reinterpret_cast<Ctx*>(ctx)->autotune = on != 0;
+  return RCWA_OK;
+}
+
+int rcwa_autotune_entries(int* out, int max_entries) {
+  return zgemm_tuned_entries(out, max_entries < 0 ? 0 : max_entries);
+}
+
+rcwa_status rcwa_zgemm_force_shape(int shape) {
+  if (shape >= zgemm_shape_count()) return rcwa_set_error(RCWA_E_ARG, "no such tile shape");
+  zgemm_force_shape(shape);
   return RCWA_OK;
 }
 

@@ -33,6 +33,7 @@ struct Ctx {
   long long qr_log = 0;  // QR rotation-log capacity per matrix (0 = default)
   int max_chunk = 0;
   int refine = 2;  // iterative refinement of the fold's solves (rcwa_ctx_set_refine)
+  bool autotune = false;  // ZGEMM tile shapes tuned on first use (rcwa_ctx_set_autotune)
   rcwa_precision precision = RCWA_FP64;
   rcwa_stats stats{};
   std::vector<std::complex<double>> last_cols;

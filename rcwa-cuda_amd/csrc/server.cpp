@@ -815,6 +815,7 @@ void Engine::fail_all(rcwa_status st, const std::string& msg) {
 
 void Engine::run() {
   (void)hipSetDevice(srv_->device);
+  ZgemmTuneScope tune(c_.autotune);
   (void)c_.set_precision(srv_->cfg.precision);  // checked at creation
   std::vector<std::unique_ptr<Flight>> retired;
   for (;;) {
@@ -916,6 +917,7 @@ rcwa_status rcwa_server_create(int device, const rcwa_server_config* cfg, rcwa_s
   for (int i = 0; i < s->cfg.n_workers; ++i) {
     auto c = std::make_unique<Ctx>();
     rcwa_status st = c->init(device, budget / s->cfg.n_workers);
+    c->autotune = s->cfg.autotune != 0;
     if (st) {
       for (auto& x : s->ctxs) x->release();
       delete s;

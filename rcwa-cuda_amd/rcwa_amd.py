@@ -171,7 +171,8 @@ class Stats(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch", ctypes.c_int),
-                ("n_workers", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("n_workers", ctypes.c_int), ("precision", ctypes.c_int),
+                ("autotune", ctypes.c_int)]
 
 
 class TicketInfo(ctypes.Structure):
@@ -183,7 +184,8 @@ class TicketInfo(ctypes.Structure):
 EXPORTS = [
     "rcwa_last_error", "rcwa_version", "rcwa_ctx_create", "rcwa_ctx_destroy",
     "rcwa_ctx_set_eig_maxit", "rcwa_ctx_set_qr_log", "rcwa_ctx_set_refine", "rcwa_ctx_get_stats", "rcwa_ctx_reset_stats", "rcwa_ctx_set_chunk",
-    "rcwa_ctx_set_timing",
+    "rcwa_ctx_set_timing", "rcwa_ctx_set_autotune", "rcwa_autotune_entries",
+    "rcwa_zgemm_force_shape",
     "rcwa_solve", "rcwa_solve_batch", "rcwa_last_source_columns", "rcwa_convmat",
     "rcwa_wave_vectors", "rcwa_gap_medium", "rcwa_layer_smatrix", "rcwa_end_smatrix",
     "rcwa_star_product", "rcwa_eig", "rcwa_inverse", "rcwa_zgemm", "rcwa_zgemm_split",
@@ -289,6 +291,9 @@ class Context:
 
     def set_refine(self, level):
         _check(lib().rcwa_ctx_set_refine(self.h, level))
+
+    def set_autotune(self, on):
+        _check(lib().rcwa_ctx_set_autotune(self.h, 1 if on else 0))
 
     def set_timing(self, on):
         _check(lib().rcwa_ctx_set_timing(self.h, 1 if on else 0))
@@ -474,6 +479,20 @@ class Context:
         return _unstack_f(C, b, m, n)
 
 
+def autotune_entries():
+    """The process's tuned ZGEMM tile shapes: (M, N, K, batch class, device,
+    shape) tuples (rcwa_autotune_entries)."""
+    n = lib().rcwa_autotune_entries(None, 0)
+    buf = (ctypes.c_int * (6 * max(n, 1)))()
+    n = lib().rcwa_autotune_entries(buf, n)
+    return [tuple(buf[6 * i:6 * i + 6]) for i in range(n)]
+
+
+def zgemm_force_shape(shape):
+    """Forces ZGEMM tile shape `shape` (0 .. 4) for the process, -1 undoes."""
+    _check(lib().rcwa_zgemm_force_shape(shape))
+
+
 # ---------------------------------------------------------------- host stages
 def wave_vectors(Lx, Ly, lam0, theta, phi, hx, hy, er_ref, ur_ref, er_trn, ur_trn):
     PQ = (2 * hx + 1) * (2 * hy + 1)
@@ -582,9 +601,9 @@ class Server:
     solves on one GPU."""
 
     def __init__(self, device=0, hbm_budget=0, max_batch=0, n_workers=0,
-                 precision=RCWA_FP64):
+                 precision=RCWA_FP64, autotune=False):
         self.h = ctypes.c_void_p()
-        cfg = ServerConfig(hbm_budget, max_batch, n_workers, precision)
+        cfg = ServerConfig(hbm_budget, max_batch, n_workers, precision, 1 if autotune else 0)
         _check(lib().rcwa_server_create(device, ctypes.byref(cfg), ctypes.byref(self.h)))
         self._meta = {}
 
