@@ -188,6 +188,151 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
   }
 }
 
+// The panel split by column for small batches (a few matrices of large n,
+// where one workgroup per matrix leaves most of the chip idle): per column
+// j = p + jj of the panel, hess_col_prep_kernel (one workgroup per matrix)
+// does steps (a) and (b) of hess_panel_kernel, stores the finished column and
+// the reflector v = V(:, jj), and forms zq = V(:, :jj)^H v (global, NB per
+// matrix) and column jj of T; hess_col_y_kernel then forms
+// Y(i, jj) = tau (A0[i, j+1:] v - Y(i, :jj) zq) for the rows p+1.. in blocks
+// of YR rows over the whole chip.  The same arithmetic per entry as the
+// panel kernel.
+constexpr int YR = 256;
+
+__global__ __launch_bounds__(PT) void hess_col_prep_kernel(int n, int p, int jj, ZMat Am, zc* Vw,
+                                                           zc* Yw, zc* Tw, zc* Zq) {
+  extern __shared__ zc sm[];
+  zc* bcol = sm;  // n
+  __shared__ zc wq[NB], w2[NB], zq[NB];
+  __shared__ double red[PT / 64];
+  __shared__ zc s_tau, s_beta, s_scale;
+  const int b = blockIdx.x;
+  zc* A = Am.p + (long long)b * Am.stride;
+  const long long lda = Am.ld;
+  zc* V = Vw + (size_t)b * n * NB;
+  zc* Y = Yw + (size_t)b * n * NB;
+  zc* T = Tw + (size_t)b * NB * NB;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = PT / 64;
+  const int j = p + jj;
+  for (int i = p + 1 + tid; i < n; i += PT) {
+    zc v = A[i + (long long)j * lda];
+    for (int q = 0; q < jj; ++q) v = zsub(v, zmul(Y[i + (size_t)q * n], zconj(V[j + (size_t)q * n])));
+    bcol[i] = v;
+  }
+  __syncthreads();
+  if (jj > 0) {
+    for (int q = wave; q < jj; q += nw) {
+      zc acc = zmake(0, 0);
+      for (int i = p + 1 + lane; i < n; i += 64)
+        acc = zadd(acc, zmul(zconj(V[i + (size_t)q * n]), bcol[i]));
+      acc = wave_sum(acc);
+      if (lane == 0) wq[q] = acc;
+    }
+    __syncthreads();
+    if (tid < jj) {
+      zc acc = zmake(0, 0);
+      for (int r = 0; r <= tid; ++r) acc = zadd(acc, zmul(zconj(T[r + tid * NB]), wq[r]));
+      w2[tid] = acc;
+    }
+    __syncthreads();
+    for (int i = p + 1 + tid; i < n; i += PT) {
+      zc v = bcol[i];
+      for (int q = 0; q < jj; ++q) v = zsub(v, zmul(V[i + (size_t)q * n], w2[q]));
+      bcol[i] = v;
+    }
+    __syncthreads();
+  }
+  double part = 0.0;
+  for (int i = j + 2 + tid; i < n; i += PT) part += zabs2(bcol[i]);
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off);
+  if (lane == 0) red[wave] = part;
+  __syncthreads();
+  if (tid == 0) {
+    double xn2 = 0.0;
+    for (int w = 0; w < nw; ++w) xn2 += red[w];
+    const zc alpha = bcol[j + 1];
+    if (xn2 == 0.0 && alpha.y == 0.0) {
+      s_tau = zmake(0, 0);
+      s_beta = alpha;
+      s_scale = zmake(0, 0);
+    } else {
+      const double nrm = sqrt(zabs2(alpha) + xn2);
+      const double beta = (alpha.x >= 0.0) ? -nrm : nrm;
+      s_tau = zmake((beta - alpha.x) / beta, -alpha.y / beta);
+      s_beta = zmake(beta, 0.0);
+      s_scale = zinv(zsub(alpha, zmake(beta, 0.0)));
+    }
+  }
+  __syncthreads();
+  const zc tau = s_tau, scale = s_scale;
+  for (int i = tid; i < n; i += PT) {
+    zc a, v;
+    if (i <= j) { a = bcol[i]; v = zmake(0, 0); }
+    else if (i == j + 1) { a = s_beta; v = zmake(1, 0); }
+    else { a = zmake(0, 0); v = zmul(bcol[i], scale); }
+    if (i > p) A[i + (long long)j * lda] = a;
+    V[i + (size_t)jj * n] = v;
+    bcol[i] = v;
+  }
+  __syncthreads();
+  for (int q = wave; q < jj; q += nw) {
+    zc acc = zmake(0, 0);
+    for (int i = j + 1 + lane; i < n; i += 64)
+      acc = zadd(acc, zmul(zconj(V[i + (size_t)q * n]), bcol[i]));
+    acc = wave_sum(acc);
+    if (lane == 0) zq[q] = acc;
+  }
+  __syncthreads();
+  zc* zg = Zq + (size_t)b * NB;
+  if (tid < jj) zg[tid] = zq[tid];
+  if (tid == jj) zg[NB - 1] = tau;  // the column's tau, for hess_col_y_kernel
+  if (tid <= jj) {
+    zc t;
+    if (tid == jj) {
+      t = tau;
+    } else {
+      zc acc = zmake(0, 0);
+      for (int r = tid; r < jj; ++r) acc = zadd(acc, zmul(T[tid + r * NB], zq[r]));
+      t = zneg(zmul(tau, acc));
+    }
+    T[tid + jj * NB] = t;
+  }
+  for (int q = jj + 1 + tid; q < NB; q += PT) T[q + jj * NB] = zmake(0, 0);
+}
+
+// Y(i, jj) for the rows i = p+1 + YR blockIdx.x + threadIdx.x: the product
+// with the trailing columns j+1.. (eight columns' loads in flight together,
+// v staged in LDS), less Y(i, :jj) zq, times tau.
+__global__ __launch_bounds__(YR) void hess_col_y_kernel(int n, int p, int jj, ZCMat Am,
+                                                        const zc* Vw, zc* Yw, const zc* Zq) {
+  extern __shared__ zc sv[];  // v[j+1 .. n-1]
+  const int b = blockIdx.y;
+  const zc* A = Am.p + (long long)b * Am.stride;
+  const long long lda = Am.ld;
+  const zc* V = Vw + (size_t)b * n * NB;
+  zc* Y = Yw + (size_t)b * n * NB;
+  const zc* zg = Zq + (size_t)b * NB;
+  const int j = p + jj;
+  for (int c = j + 1 + threadIdx.x; c < n; c += YR) sv[c - j - 1] = V[c + (size_t)jj * n];
+  __syncthreads();
+  const int i = p + 1 + blockIdx.x * YR + threadIdx.x;
+  if (i >= n) return;
+  zc y = zmake(0, 0);
+  int c = j + 1;
+  for (; c + 7 < n; c += 8) {
+    const zc* col = A + i + (long long)c * lda;
+    zc a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = col[q * lda];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) y = zfma(a[q], sv[c - j - 1 + q], y);
+  }
+  for (; c < n; ++c) y = zfma(A[i + (long long)c * lda], sv[c - j - 1], y);
+  for (int q = 0; q < jj; ++q) y = zsub(y, zmul(Y[i + (size_t)q * n], zg[q]));
+  Y[i + (size_t)jj * n] = zmul(y, zg[NB - 1]);
+}
+
 // dst (cols x rows) = src(rows x cols)^H, src column-major with ld; dst ld = cols.
 __global__ void conj_transpose_kernel(int rows, int cols, const zc* src, long long sld,
                                       long long sstride, zc* dst, long long dstride) {
@@ -227,8 +372,9 @@ int zconj_transpose_batched(hipStream_t s, int rows, int cols, ZCMat src, ZMat d
 }
 
 size_t hess_blocked_ws_bytes(int n, int batch) {
-  // V, Y (n x NB), T (NB x NB), VH (NB x n), W (NB x n), W2 (NB x n), QV (n x NB), QVT (n x NB)
-  const size_t per = (size_t)n * NB * 7 + (size_t)NB * NB * 2;
+  // V, Y (n x NB), T (NB x NB), VH (NB x n), W (NB x n), W2 (NB x n), QV (n x NB), QVT (n x NB),
+  // and zq (NB) of the column-split panel
+  const size_t per = (size_t)n * NB * 7 + (size_t)NB * NB * 2 + NB;
   return (size_t)batch * per * sizeof(zc) + 1024;
 }
 
@@ -245,13 +391,30 @@ int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, 
   zc* W2 = W + batch * nNB;
   zc* QV = W2 + batch * nNB;
   zc* QVT = QV + batch * nNB;
+  zc* ZQ = QVT + batch * nNB;
+  // a small batch takes the panel column by column over the whole chip
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const bool by_column = (long long)batch * 4 <= cus && n >= 512;
   const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
   const ZCMat none{nullptr, 0, 0};
   hipLaunchKernelGGL(identity_q_kernel, g2((long long)n * n, batch), dim3(256), 0, s, n, Q);
   for (int p = 0; p + 2 < n; p += NB) {
     const int nbe = std::min(NB, n - 2 - p);  // reflectors in this panel
-    hipLaunchKernelGGL(hess_panel_kernel, dim3(batch), dim3(PT), (size_t)n * sizeof(zc), s, n, p,
-                       nbe, A, V, Y, T);
+    if (by_column) {
+      for (int jj = 0; jj < nbe; ++jj) {
+        hipLaunchKernelGGL(hess_col_prep_kernel, dim3(batch), dim3(PT), (size_t)n * sizeof(zc), s,
+                           n, p, jj, A, V, Y, T, ZQ);
+        const int rows = n - p - 1;
+        hipLaunchKernelGGL(hess_col_y_kernel, dim3((rows + YR - 1) / YR, batch), dim3(YR),
+                           (size_t)(n - p - jj - 1) * sizeof(zc), s, n, p, jj,
+                           ZCMat{A.p, A.ld, A.stride}, V, Y, ZQ);
+      }
+    } else {
+      hipLaunchKernelGGL(hess_panel_kernel, dim3(batch), dim3(PT), (size_t)n * sizeof(zc), s, n,
+                         p, nbe, A, V, Y, T);
+    }
     const int c0 = p + nbe;                   // first column right of the panel
     const int m = n - c0;                     // trailing columns
     const int r1 = p + 1, mr = n - r1;        // rows of V that are nonzero
