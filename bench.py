@@ -15,7 +15,7 @@ The JSON line reports:
                classed as HBM-bound (no matrix-core work): its compulsory
                bytes per launch over its event-timed average duration,
                against the HBM peak, beside the traffic measured with PMC
-               counters (about 99x the compulsory bytes, moved at 2.7 TB/s:
+               counters (about 97x the compulsory bytes, moved at 2.9 TB/s:
                the phase is bound by the latency of its one-wave steps, not
                by HBM) and its FP64 flop rate
   split_sample the split-precision form on 8 of the timed wavelengths:
@@ -49,10 +49,10 @@ sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
 METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
 # HBM bytes (read + written) of the QR phase per 882x882 layer matrix, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR phase's kernels at
-# 510 matrices per launch (the mean of the step's two launches, 3151.1 and
-# 3111.0 GB), scaled by the counters' reading on a copy of known size
+# 510 matrices per launch (the mean of the step's two launches, 3096.7 and
+# 3061.4 GB), scaled by the counters' reading on a copy of known size
 # (--hbm-copy-probe; profiles/qr_phase_21x21_pmc_traffic.md).
-QR_PHASE_HBM_BYTES_PER_MATRIX = 3131.05e9 / 510
+QR_PHASE_HBM_BYTES_PER_MATRIX = 3079.05e9 / 510
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
 
