@@ -1759,10 +1759,10 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
 }
 
 // One workgroup per matrix: multishift QR with aggressive early deflation.
-// Each pass: the AED step on the trailing NW x NW window of the active block
-// (or the whole block once it fits: its Schur form is computed in LDS and the
-// block is done), then, unless it deflated more than 14% of the window, a
-// pass of nch <= NC single-shift sweeps with the bottom undeflated window
+// Each pass: the AED step on the trailing AW x AW window of the active block
+// (or the whole block once it has at most NW rows: its Schur form is computed
+// in LDS and the block is done), then, unless it deflated more than NIB% of
+// the window, a pass of nch <= NC single-shift sweeps with the bottom undeflated window
 // eigenvalues as shifts, chained GAP = 2 positions apart (chain c rotates at
 // k - c GAP at step t while chain 0 is at k), chased through the LDS band and
 // applied to the strips exactly as in schur_pipe_kernel, whose pipelining,
@@ -1772,11 +1772,12 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
 // chain, so a pass equals its sweeps one after the other, which is how the
 // rotation log records them.  The dense V of every AED step goes to the
 // block log, ordered with the sweeps (pos = sweeps logged before it).
-template <int R, int NT, int NC, int NW>
+template <int R, int NT, int NC, int NW, int AW, int NIB>
 __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, int* info,
                                                       int maxit_per_row, QrLog log, BlockLog blog) {
   static_assert(GAP_MS * (NC - 1) + 1 <= R, "chains must fit behind one window step");
   static_assert(NW <= 64 && 2 * R <= NW + 1, "AED window: one lane per row, passes beyond it");
+  static_assert(NC <= AW && AW <= NW, "the window supplies the shifts and fits the LDS area");
   constexpr int GAP = GAP_MS, GP = GAP * (NC - 1);
   constexpr int RS = R + GP + 1;   // rows spanned by a window's rotations
   constexpr int NR = NC * R;       // rotation entries per window
@@ -1860,7 +1861,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
     // active block once it fits one)
     const int mblk = iu - il + 1;
     const bool whole = mblk <= NW;
-    const int aw = whole ? mblk : NW;
+    const int aw = whole ? mblk : AW;
     const int kwtop = iu - aw + 1;
     if (tid == 0 && il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
     for (int idx = tid; idx < aw * aw; idx += NT) {
@@ -1922,13 +1923,14 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
       if (tid == 0) T[kwtop + (long long)(kwtop - 1) * ld] = s_spike;
       for (int r = il + tid; r < kwtop; r += NT) {
         zc* trow = T + r + (long long)kwtop * ld;
-        zc x[NW];
+        zc x[AW];  // aw == AW here: the window is not the whole block
 #pragma unroll
-        for (int q = 0; q < NW; ++q) x[q] = trow[q * ld];
-        for (int j = 0; j < NW; ++j) {
+        for (int q = 0; q < AW; ++q) x[q] = trow[q * ld];
+#pragma unroll 1
+        for (int j = 0; j < AW; ++j) {
           zc acc = zmake(0, 0);
 #pragma unroll
-          for (int q = 0; q < NW; ++q) acc = zadd(acc, zmul(x[q], Vw[q + j * LDW]));
+          for (int q = 0; q < AW; ++q) acc = zadd(acc, zmul(x[q], Vw[q + j * LDW]));
           trow[j * ld] = acc;
         }
       }
@@ -1952,8 +1954,9 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
     iu = kwtop + ns - 1;
     iter = nd > 0 ? 0 : iter + 1;
     // many deflations: look at the next window before sweeping (zlaqr0's
-    // nibble point, 14%); a block that now fits the window is finished there
-    if ((nd > 0 && 100 * nd > 14 * aw) || ns == 0 || iu - il + 1 <= NW) continue;
+    // nibble point, there 14%); a block that now fits NW rows is finished in
+    // the window
+    if ((nd > 0 && 100 * nd > NIB * aw) || ns == 0 || iu - il + 1 <= NW) continue;
     // as many chains as shifts, and as the rotation log still holds (at least
     // one: the check above left room for a sweep over the larger block)
     const int nch = (int)min((long long)min(NC, ns),
@@ -2425,9 +2428,14 @@ namespace {
 long long qr_log_default(int n) { return (long long)n * n + 64; }
 int qr_log_sweeps(int n) { return 8 * n + 64; }
 // The multishift iteration: windows of MS_R steps, MS_NC chains, AED windows
-// of MS_NW; room for n/3 + 32 AED blocks per matrix and launch (a C2 layer
-// matrix, n = 882, takes about 200).
-constexpr int MS_R = 16, MS_NC = 8, MS_NW = 32;
+// of MS_AW rows (1.5 shifts per row, as LAPACK's zlaqr0 sizes them for this
+// many shifts), blocks of up to MS_NW rows finished in the window, a pass
+// skipped after an AED that deflated more than MS_NIB% of its window; room
+// for n/3 + 32 AED blocks per matrix and launch.  Measured on the 21x21-
+// harmonic layers (n = 882, 510 per launch): windows of 32, 24, 20, 16, 12,
+// 10, 8 rows with the nibble at 14% took 1077, 1007, 982, 954, -, -, - ms per
+// QR phase, and 12 rows with the nibble at 25, 40, 60% 925, 918, 960 ms.
+constexpr int MS_R = 16, MS_NC = 8, MS_NW = 32, MS_AW = 12, MS_NIB = 40;
 int qr_block_cap(int n) { return n / 3 + 32; }
 }  // namespace
 
@@ -2500,7 +2508,7 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
         hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 2>), dim3(batch), dim3(256), 0, s, n, T,
                            Z, info, maxit_per_row, log);
       else
-        hipLaunchKernelGGL((schur_ms_kernel<MS_R, 256, MS_NC, MS_NW>), dim3(batch), dim3(256), 0,
+        hipLaunchKernelGGL((schur_ms_kernel<MS_R, 256, MS_NC, MS_NW, MS_AW, MS_NIB>), dim3(batch), dim3(256), 0,
                            s, n, T, Z, info, maxit_per_row, log, blog);
       if ((e = (int)hipGetLastError())) return e;
       // the log reaches the Schur vectors only: the sweeps keep the active
