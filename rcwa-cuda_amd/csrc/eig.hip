@@ -2432,9 +2432,10 @@ int qr_log_sweeps(int n) { return 8 * n + 64; }
 // many shifts), blocks of up to MS_NW rows finished in the window, a pass
 // skipped after an AED that deflated more than MS_NIB% of its window; room
 // for n/3 + 32 AED blocks per matrix and launch.  Measured on the 21x21-
-// harmonic layers (n = 882, 510 per launch): windows of 32, 24, 20, 16, 12,
-// 10, 8 rows with the nibble at 14% took 1077, 1007, 982, 954, -, -, - ms per
-// QR phase, and 12 rows with the nibble at 25, 40, 60% 925, 918, 960 ms.
+// harmonic layers (n = 882, 510 per launch), ms per QR phase: windows of 32,
+// 24, 20, 16 rows with the nibble at 14%: 1077, 1007, 982, 954; with the
+// nibble at 40%: windows of 24, 20, 16, 14, 12, 10, 8 rows: 949, 936, 939,
+// 936, 918, 942, 1011; 12 rows with the nibble at 25 and 60%: 925, 960.
 constexpr int MS_R = 16, MS_NC = 8, MS_NW = 32, MS_AW = 12, MS_NIB = 40;
 int qr_block_cap(int n) { return n / 3 + 32; }
 }  // namespace
