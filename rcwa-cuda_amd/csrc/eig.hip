@@ -1759,10 +1759,10 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
 }
 
 // One workgroup per matrix: multishift QR with aggressive early deflation.
-// Each pass: the AED step on the trailing AW x AW window of the active block
-// (or the whole block once it has at most NW rows: its Schur form is computed
-// in LDS and the block is done), then, unless it deflated more than NIB% of
-// the window, a pass of nch <= NC single-shift sweeps with the bottom undeflated window
+// Each pass: the AED step on the trailing aw x aw window of the active block
+// (aw <= NW; or the whole block once it has at most NW rows: its Schur form
+// is computed in LDS and the block is done), then, unless it deflated more
+// than nib% of the window, a pass of nch <= NC single-shift sweeps with the bottom undeflated window
 // eigenvalues as shifts, chained GAP = 2 positions apart (chain c rotates at
 // k - c GAP at step t while chain 0 is at k), chased through the LDS band and
 // applied to the strips exactly as in schur_pipe_kernel, whose pipelining,
@@ -1772,12 +1772,14 @@ __device__ zc aed_rehess(zc* Tw, zc* Vw, zc* sv, zc* hv, zc* xv, int nw, int ns,
 // chain, so a pass equals its sweeps one after the other, which is how the
 // rotation log records them.  The dense V of every AED step goes to the
 // block log, ordered with the sweeps (pos = sweeps logged before it).
-template <int R, int NT, int NC, int NW, int AW, int NIB>
+template <int R, int NT, int NC, int NW>
 __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, int* info,
-                                                      int maxit_per_row, QrLog log, BlockLog blog) {
+                                                      int maxit_per_row, QrLog log, BlockLog blog,
+                                                      int aedw, int nib) {
   static_assert(GAP_MS * (NC - 1) + 1 <= R, "chains must fit behind one window step");
   static_assert(NW <= 64 && 2 * R <= NW + 1, "AED window: one lane per row, passes beyond it");
-  static_assert(NC <= AW && AW <= NW, "the window supplies the shifts and fits the LDS area");
+  // the window supplies the shifts and fits the LDS area
+  const int AW = max(NC, min(aedw, NW));
   constexpr int GAP = GAP_MS, GP = GAP * (NC - 1);
   constexpr int RS = R + GP + 1;   // rows spanned by a window's rotations
   constexpr int NR = NC * R;       // rotation entries per window
@@ -1923,14 +1925,17 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
       if (tid == 0) T[kwtop + (long long)(kwtop - 1) * ld] = s_spike;
       for (int r = il + tid; r < kwtop; r += NT) {
         zc* trow = T + r + (long long)kwtop * ld;
-        zc x[AW];  // aw == AW here: the window is not the whole block
+        // aw == AW here (the window is not the whole block); entries past it
+        // are zero in x and never read from V
+        zc x[NW];
 #pragma unroll
-        for (int q = 0; q < AW; ++q) x[q] = trow[q * ld];
+        for (int q = 0; q < NW; ++q) x[q] = q < aw ? trow[(long long)min(q, aw - 1) * ld] : zmake(0, 0);
 #pragma unroll 1
-        for (int j = 0; j < AW; ++j) {
+        for (int j = 0; j < aw; ++j) {
           zc acc = zmake(0, 0);
 #pragma unroll
-          for (int q = 0; q < AW; ++q) acc = zadd(acc, zmul(x[q], Vw[q + j * LDW]));
+          for (int q = 0; q < NW; ++q)
+            acc = zadd(acc, zmul(x[q], q < aw ? Vw[q + j * LDW] : zmake(0, 0)));
           trow[j * ld] = acc;
         }
       }
@@ -1956,7 +1961,7 @@ __global__ __launch_bounds__(NT) void schur_ms_kernel(int n, ZMat Tm, ZMat Zm, i
     // many deflations: look at the next window before sweeping (zlaqr0's
     // nibble point, there 14%); a block that now fits NW rows is finished in
     // the window
-    if ((nd > 0 && 100 * nd > NIB * aw) || ns == 0 || iu - il + 1 <= NW) continue;
+    if ((nd > 0 && 100 * nd > nib * aw) || ns == 0 || iu - il + 1 <= NW) continue;
     // as many chains as shifts, and as the rotation log still holds (at least
     // one: the check above left room for a sweep over the larger block)
     const int nch = (int)min((long long)min(NC, ns),
@@ -2427,16 +2432,29 @@ namespace {
 // stops and resumes after its log is applied.
 long long qr_log_default(int n) { return (long long)n * n + 64; }
 int qr_log_sweeps(int n) { return 8 * n + 64; }
-// The multishift iteration: windows of MS_R steps, MS_NC chains, AED windows
-// of MS_AW rows (1.5 shifts per row, as LAPACK's zlaqr0 sizes them for this
-// many shifts), blocks of up to MS_NW rows finished in the window, a pass
-// skipped after an AED that deflated more than MS_NIB% of its window; room
-// for n/3 + 32 AED blocks per matrix and launch.  Measured on the 21x21-
-// harmonic layers (n = 882, 510 per launch), ms per QR phase: windows of 32,
-// 24, 20, 16 rows with the nibble at 14%: 1077, 1007, 982, 954; with the
-// nibble at 40%: windows of 24, 20, 16, 14, 12, 10, 8 rows: 949, 936, 939,
-// 936, 918, 942, 1011; 12 rows with the nibble at 25 and 60%: 925, 960.
-constexpr int MS_R = 16, MS_NC = 8, MS_NW = 32, MS_AW = 12, MS_NIB = 40;
+// The multishift iteration: windows of MS_R steps, MS_NC chains, blocks of
+// up to MS_NW rows finished in the deflation window, room for n/3 + 32 AED
+// blocks per matrix and launch.
+constexpr int MS_R = 16, MS_NC = 8, MS_NW = 32;
+// The AED window's rows: the window's QR costs about the same for every
+// matrix size while a pass over the active block grows with n, so larger
+// matrices take larger windows (more deflation per pass).  Measured, ms per
+// QR phase at n = 882 (21x21 harmonics, 510 matrices per launch): windows of
+// 32, 24, 20, 16 rows with the nibble at 14%: 1077, 1007, 982, 954; with the
+// nibble at 40%: 24, 20, 16, 14, 12, 10, 8 rows: 949, 936, 939, 936, 918,
+// 942, 1011; 12 rows with the nibble at 25 and 60%: 925, 960.  At n = 3362
+// (41x41 harmonics, 50 matrices) 32 rows took 8.0 s, 12 rows 8.8 s.  12
+// rows (LAPACK's zlaqr0 takes 1.5 rows per shift) up to n = 1024, 32 from
+// n = 2048, linear between.
+int ms_aed_rows(int n) {
+  if (n <= 1024) return 12;
+  if (n >= 2048) return MS_NW;
+  return 12 + (MS_NW - 12) * (n - 1024) / 1024;
+}
+// After an AED that deflated more than this share of its window (percent)
+// the pass is skipped and the next AED runs at once (zlaqr0's nibble point,
+// 14% there).
+constexpr int MS_NIB = 40;
 int qr_block_cap(int n) { return n / 3 + 32; }
 }  // namespace
 
@@ -2509,8 +2527,8 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
         hipLaunchKernelGGL((schur_pipe_kernel<12, 256, 2>), dim3(batch), dim3(256), 0, s, n, T,
                            Z, info, maxit_per_row, log);
       else
-        hipLaunchKernelGGL((schur_ms_kernel<MS_R, 256, MS_NC, MS_NW, MS_AW, MS_NIB>), dim3(batch), dim3(256), 0,
-                           s, n, T, Z, info, maxit_per_row, log, blog);
+        hipLaunchKernelGGL((schur_ms_kernel<MS_R, 256, MS_NC, MS_NW>), dim3(batch), dim3(256), 0,
+                           s, n, T, Z, info, maxit_per_row, log, blog, ms_aed_rows(n), MS_NIB);
       if ((e = (int)hipGetLastError())) return e;
       // the log reaches the Schur vectors only: the sweeps keep the active
       // block of T up to date, and eig_batched forms the whole Schur form
