@@ -15,7 +15,7 @@ The JSON line reports:
                classed as HBM-bound (no matrix-core work): its compulsory
                bytes per launch over its event-timed average duration,
                against the HBM peak, beside the traffic measured with PMC
-               counters (about 97x the compulsory bytes, moved at 2.9 TB/s:
+               counters (about 111x the compulsory bytes, moved at 3.9 TB/s:
                the phase is bound by the latency of its one-wave steps, not
                by HBM) and its FP64 flop rate
   split_sample the split-precision form on 8 of the timed wavelengths:
@@ -49,10 +49,10 @@ sys.path.insert(0, os.path.join(ROOT, "rcwa-cuda_amd"))
 METRIC = "RCWA solves/s at 21×21 orders, 20 layers; p50/p99 latency; max |ΔR| vs FP64"
 # HBM bytes (read + written) of the QR phase per 882x882 layer matrix, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the QR phase's kernels at
-# 510 matrices per launch (the mean of the step's two launches, 3096.7 and
-# 3061.4 GB), scaled by the counters' reading on a copy of known size
+# 510 matrices per launch (the mean of the step's two launches, 3513.0 and
+# 3523.8 GB), scaled by the counters' reading on a copy of known size
 # (--hbm-copy-probe; profiles/qr_phase_21x21_pmc_traffic.md).
-QR_PHASE_HBM_BYTES_PER_MATRIX = 3079.05e9 / 510
+QR_PHASE_HBM_BYTES_PER_MATRIX = 3518.4e9 / 510
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBPS = 8000.0
 
@@ -382,7 +382,7 @@ def main():
     traffic = QR_PHASE_HBM_BYTES_PER_MATRIX * per_launch_mats
     roofline = {"kernel": "QR phase: schur_ms_kernel + qr_apply_ms_kernel + 2 x zgemm_kernel "
                           "(complex multishift QR with aggressive early deflation on the "
-                          "Hessenberg matrix: 32x32 deflation windows, up to eight chained "
+                          "Hessenberg matrix: 12-row deflation windows at this size, up to eight chained "
                           "single-shift sweeps per pass; the logged rotations and window "
                           "blocks applied to the Schur vectors Z; Schur form T = Z^H A Z)",
                 "bound": "hbm", "achieved": gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
