@@ -28,7 +28,6 @@
 namespace {
 
 constexpr int HESS_THREADS = 1024;
-constexpr int QR_THREADS = 512;
 // Rotation logs of a batch of QR iterations (per matrix: room for cap
 // rotations and scap sweeps, of which at most limit rotations are used before
 // the sweep kernel applies the log itself; nsw[b] = sweeps logged and not yet
@@ -189,292 +188,18 @@ __device__ zc wilkinson(zc a, zc b, zc c, zc d) {
   return zsub(d, zdiv(bc, den));
 }
 
-__global__ __launch_bounds__(QR_THREADS) void schur_kernel(int n, ZMat Tm, ZMat Zm, int* info,
-                                                           int maxit_per_row) {
-  __shared__ int s_il;
-  __shared__ double s_c;
-  __shared__ zc s_s, s_mu;
-  const int b = blockIdx.x;
-  zc* T = Tm.p + (long long)b * Tm.stride;
-  zc* Z = Zm.p + (long long)b * Zm.stride;
-  const long long ld = Tm.ld, ldz = Zm.ld;
-  const int tid = threadIdx.x;
-  const double eps = DBL_EPSILON;
-  const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
-  int iu = n - 1, iter = 0;
-  long long total = 0;
-  while (iu > 0) {
-    if (tid == 0) s_il = 0;
-    __syncthreads();
-    for (int k = iu - tid; k >= 1; k -= QR_THREADS) {
-      const double sub = zabs1(T[k + (k - 1) * ld]);
-      const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
-      if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
-    }
-    __syncthreads();
-    const int il = s_il;
-    if (il == iu) {
-      if (tid == 0) T[iu + (iu - 1) * ld] = zmake(0, 0);
-      --iu;
-      iter = 0;
-      __syncthreads();
-      continue;
-    }
-    ++iter;
-    ++total;
-    if (total > maxit) {
-      if (tid == 0 && info) info[b] = 1;
-      break;
-    }
-    if (tid == 0) {
-      if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
-      zc mu;
-      if (iter == 10)
-        mu = zadd(T[il + il * ld], zmake(0.75 * fabs(T[(il + 1) + il * ld].x), 0));
-      else if (iter == 20)
-        mu = zadd(T[iu + iu * ld], zmake(0.75 * fabs(T[iu + (iu - 1) * ld].x), 0));
-      else
-        mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
-                       T[iu + (iu - 1) * ld], T[iu + iu * ld]);
-      s_mu = mu;
-    }
-    __syncthreads();
-    for (int k = il; k < iu; ++k) {
-      if (tid == 0) {
-        zc f, g;
-        if (k == il) {
-          f = zsub(T[il + il * ld], s_mu);
-          g = T[(il + 1) + il * ld];
-        } else {
-          f = T[k + (k - 1) * ld];
-          g = T[(k + 1) + (k - 1) * ld];
-        }
-        double c;
-        zc s, r;
-        givens(f, g, c, s, r);
-        if (k > il) {
-          T[k + (k - 1) * ld] = r;
-          T[(k + 1) + (k - 1) * ld] = zmake(0, 0);
-        }
-        s_c = c;
-        s_s = s;
-      }
-      __syncthreads();
-      const double c = s_c;
-      const zc s = s_s, sc = zconj(s);
-      for (int j = k + tid; j < n; j += QR_THREADS) {
-        const zc x = T[k + j * ld], y = T[(k + 1) + j * ld];
-        T[k + j * ld] = zadd(zscale(x, c), zmul(s, y));
-        T[(k + 1) + j * ld] = zsub(zscale(y, c), zmul(sc, x));
-      }
-      for (int i = tid; i < n; i += QR_THREADS) {
-        const zc x = Z[i + k * ldz], y = Z[i + (k + 1) * ldz];
-        Z[i + k * ldz] = zadd(zscale(x, c), zmul(sc, y));
-        Z[i + (k + 1) * ldz] = zsub(zscale(y, c), zmul(s, x));
-      }
-      __syncthreads();
-      const int ie = min(k + 2, iu);
-      for (int i = tid; i <= ie; i += QR_THREADS) {
-        const zc x = T[i + k * ld], y = T[i + (k + 1) * ld];
-        T[i + k * ld] = zadd(zscale(x, c), zmul(sc, y));
-        T[i + (k + 1) * ld] = zsub(zscale(y, c), zmul(s, x));
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Windowed variant of the QR sweep.  A sweep over the active block [il, iu]
-// is cut into windows of R consecutive Givens rotations.  Per window:
-//   1. the diagonal band rows/cols [k0-1, k0+R+1] of T is staged in LDS;
-//   2. wave 0 chases the bulge through the band: it computes the R
-//      rotations and applies each one to the band only (rows, then columns,
-//      in the sequential order of the unwindowed sweep);
-//   3. all threads write the band back and apply the R rotations, held in
-//      LDS, to everything outside the band: the rows [k0, k0+R] right of the
-//      band (one thread per column, its R+1 entries in registers), the
-//      columns [k0, k0+R] above the band and the same columns of Z (one
-//      thread per row).  Each element outside the band is read and written
-//      once per window instead of once per rotation.
-// The arithmetic per element and the rotation order are those of the
-// unwindowed sweep.
-template <int R, int NT>
-__global__ __launch_bounds__(NT) void schur_win_kernel(int n, ZMat Tm, ZMat Zm, int* info,
-                                                       int maxit_per_row) {
-  constexpr int LDB = R + 3;
-  __shared__ zc band[LDB * LDB];
-  __shared__ double rc[R];
-  __shared__ zc rs[R];
-  __shared__ int s_il;
-  __shared__ zc s_mu;
-  const int b = blockIdx.x;
-  zc* T = Tm.p + (long long)b * Tm.stride;
-  zc* Z = Zm.p + (long long)b * Zm.stride;
-  const long long ld = Tm.ld, ldz = Zm.ld;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const double eps = DBL_EPSILON;
-  const long long maxit = (long long)maxit_per_row * (n > 1 ? n : 1);
-  int iu = n - 1, iter = 0;
-  long long total = 0;
-  while (iu > 0) {
-    if (tid == 0) s_il = 0;
-    __syncthreads();
-    for (int k = iu - tid; k >= 1; k -= NT) {
-      const double sub = zabs1(T[k + (k - 1) * ld]);
-      const double tst = zabs1(T[(k - 1) + (k - 1) * ld]) + zabs1(T[k + k * ld]);
-      if (sub <= eps * tst || sub < DBL_MIN) atomicMax(&s_il, k);
-    }
-    __syncthreads();
-    const int il = s_il;
-    if (il == iu) {
-      if (tid == 0) T[iu + (iu - 1) * ld] = zmake(0, 0);
-      --iu;
-      iter = 0;
-      __syncthreads();
-      continue;
-    }
-    ++iter;
-    ++total;
-    if (total > maxit) {
-      if (tid == 0 && info) info[b] = 1;
-      break;
-    }
-    if (tid == 0) {
-      if (il > 0) T[il + (il - 1) * ld] = zmake(0, 0);
-      zc mu;
-      if (iter == 10)
-        mu = zadd(T[il + il * ld], zmake(0.75 * fabs(T[(il + 1) + il * ld].x), 0));
-      else if (iter == 20)
-        mu = zadd(T[iu + iu * ld], zmake(0.75 * fabs(T[iu + (iu - 1) * ld].x), 0));
-      else
-        mu = wilkinson(T[(iu - 1) + (iu - 1) * ld], T[(iu - 1) + iu * ld],
-                       T[iu + (iu - 1) * ld], T[iu + iu * ld]);
-      s_mu = mu;
-    }
-    __syncthreads();
-    for (int k0 = il; k0 < iu;) {
-      const int k1 = min(k0 + R, iu);          // rotations k0 .. k1-1
-      const int nr = k1 - k0;
-      const int b0 = k0 > 0 ? k0 - 1 : 0;      // band rows/cols [b0, be]
-      const int be = min(k1 + 1, n - 1);
-      const int nb = be - b0 + 1;
-      for (int idx = tid; idx < nb * nb; idx += NT) {
-        const int i = idx % nb, j = idx / nb;
-        band[i + j * LDB] = T[(b0 + i) + (long long)(b0 + j) * ld];
-      }
-      __syncthreads();
-      if (wave == 0) {
-        for (int k = k0; k < k1; ++k) {
-          double c = 0.0;
-          zc sr = zmake(0, 0);
-          if (lane == 0) {
-            zc f, g, r;
-            if (k == il) {
-              f = zsub(band[(k - b0) + (k - b0) * LDB], s_mu);
-              g = band[(k + 1 - b0) + (k - b0) * LDB];
-            } else {
-              f = band[(k - b0) + (k - 1 - b0) * LDB];
-              g = band[(k + 1 - b0) + (k - 1 - b0) * LDB];
-            }
-            givens(f, g, c, sr, r);
-            if (k > il) {
-              band[(k - b0) + (k - 1 - b0) * LDB] = r;
-              band[(k + 1 - b0) + (k - 1 - b0) * LDB] = zmake(0, 0);
-            }
-            rc[k - k0] = c;
-            rs[k - k0] = sr;
-          }
-          c = __shfl(c, 0);
-          sr = zmake(__shfl(sr.x, 0), __shfl(sr.y, 0));
-          const zc sc = zconj(sr);
-          // rows k, k+1 over band columns k .. be
-          for (int j = k + lane; j <= be; j += 64) {
-            zc* p0 = &band[(k - b0) + (j - b0) * LDB];
-            const zc x = p0[0], y = p0[1];
-            p0[0] = zadd(zscale(x, c), zmul(sr, y));
-            p0[1] = zsub(zscale(y, c), zmul(sc, x));
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          // columns k, k+1 over band rows b0 .. min(k+2, iu)
-          const int ie = min(k + 2, iu);
-          for (int i = b0 + lane; i <= ie; i += 64) {
-            zc* q0 = &band[(i - b0) + (k - b0) * LDB];
-            const zc x = q0[0], y = q0[LDB];
-            q0[0] = zadd(zscale(x, c), zmul(sc, y));
-            q0[LDB] = zsub(zscale(y, c), zmul(sr, x));
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-        }
-      }
-      __syncthreads();
-      // band back to T
-      for (int idx = tid; idx < nb * nb; idx += NT) {
-        const int i = idx % nb, j = idx / nb;
-        T[(b0 + i) + (long long)(b0 + j) * ld] = band[i + j * LDB];
-      }
-      // rows k0..k1 right of the band: one column per thread
-      for (int j = be + 1 + tid; j < n; j += NT) {
-        zc v[R + 1];
-        zc* col = T + k0 + (long long)j * ld;
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) v[q] = col[q];
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          if (q < nr) {
-            const double c = rc[q];
-            const zc sr = rs[q], sc = zconj(sr);
-            const zc x = v[q], y = v[q + 1];
-            v[q] = zadd(zscale(x, c), zmul(sr, y));
-            v[q + 1] = zsub(zscale(y, c), zmul(sc, x));
-          }
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) col[q] = v[q];
-      }
-      // columns k0..k1 above the band (rows 0..b0-1 of T) and of Z (all rows)
-      for (int i = tid; i < b0 + n; i += NT) {
-        zc* base;
-        long long lds;
-        if (i < b0) { base = T + i + (long long)k0 * ld; lds = ld; }
-        else { base = Z + (i - b0) + (long long)k0 * ldz; lds = ldz; }
-        zc v[R + 1];
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) v[q] = base[q * lds];
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-          if (q < nr) {
-            const double c = rc[q];
-            const zc sr = rs[q], sc = zconj(sr);
-            const zc x = v[q], y = v[q + 1];
-            v[q] = zadd(zscale(x, c), zmul(sc, y));
-            v[q + 1] = zsub(zscale(y, c), zmul(sr, x));
-          }
-#pragma unroll
-        for (int q = 0; q <= R; ++q)
-          if (q <= nr) base[q * lds] = v[q];
-      }
-      __syncthreads();
-      k0 = k1;
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // QR sweeps with deferred updates.  The iteration itself only ever reads the
 // active block T[il:iu+1, il:iu+1] (the block only shrinks, or moves to rows
 // and columns that no sweep since its last visit has touched), so each sweep
-// updates just that block (windowed as in schur_win_kernel) and logs its
-// rotations.  Everything else the sweeps change -- Z, the rows of T above the
+// updates just that block (in windows of R rotations staged in LDS) and logs
+// its rotations.  Everything else the sweeps change -- Z, the rows of T above the
 // block and the columns of T right of it -- is applied after the iteration by
 // qr_apply_kernel, which spreads the rows and columns of every matrix over the
 // whole chip (or in the sweep kernel itself when its log fills up).  A pass
 // applies G logged sweeps at once in wavefront order: rotation k of sweep a
 // acts at time tau = k + 2a, so one pass over a row (or column) applies the G
-// sweeps in the order of the unwindowed algorithm while holding only 2G
+// sweeps in the order of the one-sweep-at-a-time algorithm while holding only 2G
 // consecutive entries, in a circular register window whose slot indices are
 // static inside an unrolled block of 2G time steps.  Each element outside the
 // active block is read and written once per G sweeps.
@@ -592,7 +317,7 @@ __device__ void fill_table(const SweepGroup<G>& g, const QrLogM& L, int T0, doub
 // c = lo..hi, for the chunk T0 .. T1 of the group's time range, as column
 // rotations read from the LDS table.  Rotation k of sweep a acts at time
 // tau = k + 2a, so one
-// pass applies the group in the order of the unwindowed algorithm while
+// pass applies the group in the order of the one-sweep-at-a-time algorithm while
 // holding only the entries in flight, in a circular register window (slot of
 // column c: c mod W) whose slot indices are static inside an unrolled block of
 // W time steps.  Every step of a block runs unconditionally -- a time with no
@@ -2423,9 +2148,6 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch) {
   return (int)hipGetLastError();
 }
 
-int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
-                             int maxit_per_row);
-
 namespace {
 // n^2 rotations: the 21x21-harmonic layers need about 0.8 n^2 (1.6 sweeps per
 // eigenvalue over an average active block of n/2); a matrix that needs more
@@ -2474,15 +2196,13 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
   // (schur_ms_kernel).  RCWA_QR_SWEEP (read on every call) selects another
   // form of the QR iteration, for tests and comparisons: =pipe the pipelined
   // sweep with two chained sweeps and one Wilkinson shift per pass, =single
-  // the same with one sweep, =direct the unwindowed sweep, =window the
-  // windowed sweep that updates all of T and Z at once, =oneshot the logged
-  // sweep without the pipelined wave 0 (also used for matrices of 2^31 bytes
-  // or more, which the pipelined kernels' 32-bit buffer offsets cannot
-  // address).
+  // the same with one sweep, =oneshot the logged sweep without the pipelined
+  // wave 0 (also used for matrices of 2^31 bytes or more, which the pipelined
+  // kernels' 32-bit buffer offsets cannot address).
   const char* mode_env = getenv("RCWA_QR_SWEEP");
   const char mode = mode_env ? mode_env[0] : 0;
-  if (mode == 'd') return schur_batched_unwindowed(s, n, T, Z, batch, info, maxit_per_row);
-  if (ws && mode != 'w') {
+  if (!ws) return (int)hipErrorInvalidValue;
+  {
     auto align = [](uintptr_t p) { return (p + 255) & ~(uintptr_t)255; };
     const long long cap_full = qr_log_default(n);
     const int scap = qr_log_sweeps(n);
@@ -2551,17 +2271,6 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
       if ((e = (int)hipGetLastError())) return e;
     }
   }
-  hipLaunchKernelGGL((schur_win_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z, info,
-                     maxit_per_row);
-  return (int)hipGetLastError();
-}
-
-int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
-                             int maxit_per_row) {
-  if (n <= 0 || batch <= 0) return 0;
-  hipLaunchKernelGGL(schur_kernel, dim3(batch), dim3(QR_THREADS), 0, s, n, T, Z, info,
-                     maxit_per_row);
-  return (int)hipGetLastError();
 }
 
 int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch, bool unit) {

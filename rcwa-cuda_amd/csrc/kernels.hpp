@@ -81,15 +81,12 @@ int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
 // Blocked (compact-WY panels + ZGEMM updates); ws holds hess_blocked_ws_bytes(n, batch).
 size_t hess_blocked_ws_bytes(int n, int batch);
 int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, void* ws);
-// ws (schur_ws_bytes) enables the deferred-update kernels (sweeps on the
-// active block with a rotation log, then the log applied to Z and the rest of
-// T); null selects the windowed kernel that updates all of T and Z per window.
+// ws (schur_ws_bytes, required) holds the rotation and block logs: the sweeps
+// run on the active block and log their rotations, which are then applied to
+// the Schur vectors Z.
 size_t schur_ws_bytes(int n, int batch);
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row,
-                  void* ws = nullptr, long long log_cap = 0);
-// The same sweeps with every rotation applied directly in global memory.
-int schur_batched_unwindowed(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
-                             int maxit_per_row);
+                  void* ws, long long log_cap = 0);
 // unit: T has a unit diagonal; X = T^-1 (no shift).
 int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch, bool unit = false);
 
