@@ -496,6 +496,70 @@ int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* i
   return (int)hipGetLastError();
 }
 
+namespace {
+
+constexpr int BLK_INV_BASE = 1024;  // blocks this size or smaller: Gauss-Jordan
+constexpr int BLK_INV_MAXB = 8;     // larger batches keep Gauss-Jordan busy enough
+
+bool use_block_inverse(int n, int batch) { return n > BLK_INV_BASE && batch <= BLK_INV_MAXB; }
+
+// entries per matrix of the recursion's work buffers: per level n1 x n2 for
+// A^-1 B and n2 x n1 for C A^-1 (n1 = m / 2), the deeper levels after them
+size_t block_inverse_rec_entries(int m) {
+  size_t z = 0;
+  while (m > BLK_INV_BASE) {
+    const int n1 = m / 2, n2 = m - n1;
+    z += 2 * (size_t)n1 * n2;
+    m = n2;  // the larger half: the inverse of A (n1) runs in the same space first
+  }
+  return z;
+}
+
+// M = [A B; C D] (A n1 x n1) in place:
+//   A <- A^-1 (recursively), W = A^-1 B, D <- S = D - C W, D <- S^-1
+//   (recursively), V = C A^-1, B <- -W S^-1, A <- A^-1 - B V, C <- -S^-1 V,
+// which is M^-1 = [A^-1 + A^-1 B S^-1 C A^-1, -A^-1 B S^-1; -S^-1 C A^-1,
+// S^-1].
+int block_inverse_rec(hipStream_t s, int m, ZMat M, int batch, void* gjws, zc* rws, int* info) {
+  if (m <= BLK_INV_BASE) return gj_inverse_batched(s, m, M, batch, gjws, info);
+  const int n1 = m / 2, n2 = m - n1;
+  const long long ld = M.ld, st = M.stride;
+  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const ZCMat none{nullptr, 0, 0};
+  const ZMat A{M.p, ld, st}, B{M.p + (long long)n1 * ld, ld, st}, C{M.p + n1, ld, st},
+      D{M.p + n1 + (long long)n1 * ld, ld, st};
+  auto cm = [](const ZMat& x) { return ZCMat{x.p, x.ld, x.stride}; };
+  const long long blk = (long long)n1 * n2;
+  const ZMat W{rws, n1, blk}, V{rws + (size_t)batch * blk, n2, blk};
+  zc* sub = rws + 2 * (size_t)batch * blk;
+  int e;
+  if ((e = block_inverse_rec(s, n1, A, batch, gjws, sub, info))) return e;
+  if ((e = zgemm_batched(s, n1, n2, n1, one, cm(A), cm(B), zero, none, W, batch))) return e;
+  if ((e = zgemm_batched(s, n2, n2, n1, mone, cm(C), cm(W), one, cm(D), D, batch))) return e;
+  if ((e = block_inverse_rec(s, n2, D, batch, gjws, sub, info))) return e;
+  if ((e = zgemm_batched(s, n2, n1, n1, one, cm(C), cm(A), zero, none, V, batch))) return e;
+  if ((e = zgemm_batched(s, n1, n2, n2, mone, cm(W), cm(D), zero, none, B, batch))) return e;
+  if ((e = zgemm_batched(s, n1, n1, n2, mone, cm(B), cm(V), one, cm(A), A, batch))) return e;
+  return zgemm_batched(s, n2, n1, n2, mone, cm(D), cm(V), zero, none, C, batch);
+}
+
+}  // namespace
+
+size_t block_inverse_ws_bytes(int n, int cap) {
+  size_t b = gj_workspace_bytes(n, cap);
+  if (n > BLK_INV_BASE)
+    b += 256 + (size_t)std::min(cap, BLK_INV_MAXB) * block_inverse_rec_entries(n) * sizeof(zc);
+  return b;
+}
+
+int block_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* info) {
+  if (!use_block_inverse(n, batch)) return gj_inverse_batched(s, n, M, batch, ws, info);
+  // the Gauss-Jordan part for this batch first, then the recursion's buffers:
+  // both fit block_inverse_ws_bytes(n, cap) for batch <= cap
+  zc* rws = (zc*)(((uintptr_t)ws + gj_workspace_bytes(n, batch) + 255) & ~(uintptr_t)255);
+  return block_inverse_rec(s, n, M, batch, ws, rws, info);
+}
+
 int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag) {
   hipLaunchKernelGGL(identity_kernel, grid_for((long long)n * n, batch), dim3(256), 0, s, n, M,
                      diag);

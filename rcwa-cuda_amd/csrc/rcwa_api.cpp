@@ -432,7 +432,7 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
   z += 13 * (size_t)G * n2;                         // fold slots
   z += (size_t)G * 32 * N + (size_t)G * 4 * n;      // end diagonals, gathered columns
   size_t bytes = z * sizeof(zc) + T1 * (2 * sizeof(double) + 3 * sizeof(int)) + 64;
-  bytes += layer_work_bytes(N, Bl) + gj_workspace_bytes((int)n, std::max(G, 1)) +
+  bytes += layer_work_bytes(N, Bl) + block_inverse_ws_bytes((int)n, std::max(G, 1)) +
            eig_workspace_bytes((int)n, Bl) + 64 * 256;
   return bytes;
 }
@@ -522,7 +522,7 @@ rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool 
   const int refine = c->refine;
   CK(gemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
   if (refine > 0) CK(zcopy_batched(fs, n, n, CMt(Fm), Mt(Mf), na));
-  CK(gj_inverse_batched(fs, n, Mt(Fm), na, gj, info));
+  CK(block_inverse_batched(fs, n, Mt(Fm), na, gj, info));
   // the columns, with the old P12 and P22
   CK(gemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
   CK(gemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
@@ -680,7 +680,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   double* dcond = (double*)c.take(T1 * sizeof(double));
   int* dnonfin = (int*)c.take(sizeof(int));
   int* dcolidx = (int*)c.take(4 * sizeof(int));
-  void* gjf = c.take(gj_workspace_bytes(n, std::max(G, 1)));
+  void* gjf = c.take(block_inverse_ws_bytes(n, std::max(G, 1)));  // the fold's inverses (and the row inverses)
   LayerWork w;
   w.carve(c, N, Bl);
   zc *dcoef_e = nullptr, *dcoef_u = nullptr, *erci_all = nullptr, *urci_all = nullptr;

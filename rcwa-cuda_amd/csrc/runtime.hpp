@@ -106,7 +106,7 @@ struct Fold {
   zc *P12 = nullptr, *P22 = nullptr, *P12n = nullptr, *P22n = nullptr, *Fm = nullptr,
      *Gm = nullptr, *Hm = nullptr, *Id = nullptr, *Mf = nullptr, *vu = nullptr, *X11 = nullptr,
      *X21 = nullptr, *va = nullptr, *vb = nullptr;
-  void* gj = nullptr;
+  void* gj = nullptr;        // block_inverse_ws_bytes(n, G) bytes
   int* info = nullptr;       // singular-pivot flags
   const int* colidx = nullptr;  // device {N/2, N/2 + N}
   int gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch);

@@ -215,7 +215,7 @@ size_t session_bytes(int N, int Nx, int Ny, int hx, int hy, int S, int B, int Bl
   bytes += (size_t)B * sizeof(double) + (size_t)B * sizeof(int);
   bytes += 2 * (2 * (size_t)B * sizeof(int) + (size_t)B * sizeof(double) +
                 index_ints(S) * sizeof(int) + 2 * (size_t)S * sizeof(int));
-  bytes += layer_work_bytes(N, Bl) + gj_workspace_bytes((int)n, S) + 64 * 256;
+  bytes += layer_work_bytes(N, Bl) + block_inverse_ws_bytes((int)n, S) + 64 * 256;
   return bytes;
 }
 
@@ -343,7 +343,7 @@ rcwa_status Engine::open_session(int hx, int hy, int Nx, int Ny) {
   f.vu = Z(3 * S * 2 * n);
   f.va = Z(S * 2 * n);
   f.vb = Z(S * 2 * n);
-  f.gj = c_.take(gj_workspace_bytes(se.n, S));
+  f.gj = c_.take(block_inverse_ws_bytes(se.n, S));
   se.w.carve(c_, se.N, B);
   se.dcolidx = (int*)c_.take(4 * sizeof(int));
   if (rcwa_status st = c_.arena_check()) return st;
