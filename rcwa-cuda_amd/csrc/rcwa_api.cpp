@@ -188,7 +188,8 @@ size_t layer_work_bytes(int N, int Bl) {
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
-                        ZCMat erci_pre, ZCMat urci_pre, double* cond) {
+                        ZCMat erci_pre, ZCMat urci_pre, double* cond, int parts,
+                        const std::function<rcwa_status(int)>& on_part) {
   const int n = 2 * N;
   const long long n2 = (long long)n * n, N2 = (long long)N * N;
   hipStream_t s = c.stream;
@@ -249,27 +250,49 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // a 2x2 block of diagonals) -> b2; W^-1 is in b3; Y = W^-1 P V0 -> b0 (P
   // is not needed after this), FP64 in both precision forms (it carries the
   // eigenvectors); A -> b0, B -> b1.
-  CK(mul_v0(s, N, CM(b0), v0, M(b2), Bl));
-  CK(zgemm_batched(s, n, n, n, one, CM(b3), CM(b2), zero, none, M(b0), Bl));
-  CK(build_ab(s, n, CM(b3), M(b0), ilam, M(b1), Bl));
-  // A^-1
-  CK(zcopy_batched(s, n, n, CM(b0), M(b2), Bl));
-  CK(gj_inverse_batched(s, n, M(b2), Bl, w.gj, info_sing));
-  // B A^-1 -> b3; C2 = A - (B A^-1) B -> b2 (rcwa.cpp:348, the reference's
-  // B (A^-1 B) reassociated so that B A^-1 serves twice; A^-1 is not needed
-  // after this)
-  CK(c.gemm(n, n, n, one, CM(b1), CM(b2), zero, none, M(b3), Bl));
-  CK(c.gemm(n, n, n, mone, CM(b3), CM(b1), one, CM(b0), M(b2), Bl));
-  // M1 = X B A^-1 X (in place)
-  CK(zscale_rc(s, n, n, M(b3), X, n, X, n, Bl));
-  // D = A - M1 B ; R11 = M1 A - B (rcwa.cpp:346-347)
-  CK(c.gemm(n, n, n, mone, CM(b3), CM(b1), one, CM(b0), M(b4), Bl));
-  CK(c.gemm(n, n, n, one, CM(b3), CM(b0), mone, CM(b1), M(b5), Bl));
-  // S11 = D^-1 R11 ; S12 = D^-1 X C2
-  CK(gj_inverse_batched(s, n, M(b4), Bl, w.gj, info_sing));
-  CK(c.gemm(n, n, n, one, CM(b4), CM(b5), zero, none, S11, Bl));
-  CK(zscale_rc(s, n, n, M(b4), nullptr, 0, X, n, Bl));
-  CK(c.gemm(n, n, n, one, CM(b4), CM(b2), zero, none, S12, Bl));
+  // From here on every matrix is independent of the others: with parts > 1
+  // the batch is finished in that many sub-batches, and on_part(done) runs
+  // after each (done = layers of the batch with their S-matrices formed), so
+  // that a caller can queue work on the first layers while the last are
+  // formed.
+  const int np = std::max(1, std::min(parts, Bl));
+  for (int q = 0; q < np; ++q) {
+    const int j0 = (int)((long long)Bl * q / np), j1 = (int)((long long)Bl * (q + 1) / np);
+    const int m = j1 - j0;
+    if (m <= 0) continue;
+    const size_t o = (size_t)j0 * n2;
+    auto Mq = [&](zc* p) { return ZMat{p + o, n, n2}; };
+    auto CMq = [&](const zc* p) { return ZCMat{p + o, n, n2}; };
+    const zc* ilq = ilam + (size_t)j0 * n;
+    const zc* Xq = X + (size_t)j0 * n;
+    int* isq = info_sing + j0;
+    const ZMat S11q{S11.p + (size_t)j0 * S11.stride, S11.ld, S11.stride};
+    const ZMat S12q{S12.p + (size_t)j0 * S12.stride, S12.ld, S12.stride};
+    CK(mul_v0(s, N, CMq(b0), v0 + (size_t)j0 * 4 * N, Mq(b2), m));
+    CK(zgemm_batched(s, n, n, n, one, CMq(b3), CMq(b2), zero, none, Mq(b0), m));
+    CK(build_ab(s, n, CMq(b3), Mq(b0), ilq, Mq(b1), m));
+    // A^-1
+    CK(zcopy_batched(s, n, n, CMq(b0), Mq(b2), m));
+    CK(gj_inverse_batched(s, n, Mq(b2), m, w.gj, isq));
+    // B A^-1 -> b3; C2 = A - (B A^-1) B -> b2 (rcwa.cpp:348, the reference's
+    // B (A^-1 B) reassociated so that B A^-1 serves twice; A^-1 is not needed
+    // after this)
+    CK(c.gemm(n, n, n, one, CMq(b1), CMq(b2), zero, none, Mq(b3), m));
+    CK(c.gemm(n, n, n, mone, CMq(b3), CMq(b1), one, CMq(b0), Mq(b2), m));
+    // M1 = X B A^-1 X (in place)
+    CK(zscale_rc(s, n, n, Mq(b3), Xq, n, Xq, n, m));
+    // D = A - M1 B ; R11 = M1 A - B (rcwa.cpp:346-347)
+    CK(c.gemm(n, n, n, mone, CMq(b3), CMq(b1), one, CMq(b0), Mq(b4), m));
+    CK(c.gemm(n, n, n, one, CMq(b3), CMq(b0), mone, CMq(b1), Mq(b5), m));
+    // S11 = D^-1 R11 ; S12 = D^-1 X C2
+    CK(gj_inverse_batched(s, n, Mq(b4), m, w.gj, isq));
+    CK(c.gemm(n, n, n, one, CMq(b4), CMq(b5), zero, none, S11q, m));
+    CK(zscale_rc(s, n, n, Mq(b4), nullptr, 0, Xq, n, m));
+    CK(c.gemm(n, n, n, one, CMq(b4), CMq(b2), zero, none, S12q, m));
+    if (on_part) {
+      if (rcwa_status st = on_part(j1)) return st;
+    }
+  }
   return RCWA_OK;
 }
 
@@ -875,6 +898,22 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   // long single solves; the wait it adds is one stream synchronisation per
   // batch)
   const bool progress = getenv("RCWA_PROGRESS") != nullptr;
+  // fold steps whose layers are among the first `upto` layer matrices, queued
+  // on the fold stream behind the layer stage's work so far
+  auto queue_fold = [&](int upto) -> rcwa_status {
+    if (lf >= Lmax || off[lf + 1] > upto) return RCWA_OK;
+    if (rcwa_status e = fence(s, fs)) return e;
+    for (; lf < Lmax && off[lf + 1] <= upto; ++lf) {
+      const ZCMat x11 = CMt(S11l + (size_t)off[lf] * n2), x12 = CMt(S12l + (size_t)off[lf] * n2);
+      if (rcwa_status e = step(act[lf], x11, x12, x12, x11, false))  // S21 = S12, S22 = S11
+        return e;
+    }
+    return RCWA_OK;
+  };
+  // With the fold on its own stream, each layer batch is finished (after its
+  // eigensolver) in two halves, and the fold steps of the first half's
+  // layers start while the second half is formed.
+  const int parts = overlap ? 2 : 1;
   for (int l0 = 0; l0 < TL; l0 += Bl) {
     const int b = std::min(Bl, TL - l0);
     // sweep points: inverses formed per row above; [[mu]] = I is its own
@@ -888,21 +927,15 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
                      dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
                      ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
                      ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
-                     info_eig + l0, info_sing + l0, erci_pre, urci_pre, dcond + l0);
+                     info_eig + l0, info_sing + l0, erci_pre, urci_pre, dcond + l0, parts,
+                     [&](int done) { return queue_fold(l0 + done); });
     if (st) return st;
     if (progress) {
       CKH(hipStreamSynchronize(s));
       fprintf(stderr, "[rcwa] layer matrices %d/%d done\n", l0 + b, TL);
       fflush(stderr);
     }
-    if (lf < Lmax && off[lf + 1] <= l0 + b) {
-      if ((st = fence(s, fs)) != RCWA_OK) return st;
-      for (; lf < Lmax && off[lf + 1] <= l0 + b; ++lf) {
-        const ZCMat x11 = CMt(S11l + (size_t)off[lf] * n2), x12 = CMt(S12l + (size_t)off[lf] * n2);
-        st = step(act[lf], x11, x12, x12, x11, false);  // layers: S21 = S12, S22 = S11
-        if (st) return st;
-      }
-    }
+    if ((st = queue_fold(l0 + b)) != RCWA_OK) return st;
   }
   CKH(hipEventRecord(c.ev[4], s));
   // * S_trn: only the columns

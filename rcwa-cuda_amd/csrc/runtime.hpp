@@ -3,6 +3,7 @@
 #include <string>
 #include <vector>
 #include <complex>
+#include <functional>
 
 #include "rcwa_amd.h"
 #include "common.hpp"
@@ -77,12 +78,15 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
 
 // erci_pre / urci_pre: [[eps]]^-1 / [[mu]]^-1 already formed (p != nullptr),
 // else the stage inverts erc / urc itself.  cond (optional, Bl doubles)
-// receives sqrt(n) ||W^-1||_F per layer, a bound on cond_2(W).
+// receives sqrt(n) ||W^-1||_F per layer, a bound on cond_2(W).  After the
+// eigensolver the batch may be finished in `parts` sub-batches, on_part(done)
+// running after each with the number of layers whose S-matrices are formed.
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
                         ZCMat erci_pre = ZCMat{nullptr, 0, 0},
-                        ZCMat urci_pre = ZCMat{nullptr, 0, 0}, double* cond = nullptr);
+                        ZCMat urci_pre = ZCMat{nullptr, 0, 0}, double* cond = nullptr,
+                        int parts = 1, const std::function<rcwa_status(int)>& on_part = nullptr);
 
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
                          void* gjws, int* info_sing, bool keep_slots);
