@@ -889,8 +889,62 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   fold.colidx = dcolidx;
   if ((st = fold.start(dends, 32LL * N, G)) != RCWA_OK) return st;
   auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };
-  auto step = [&](int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool columns_only) {
-    return fold.step(na, x11, x12, x21, x22, columns_only);
+  // A fold step over a layer launches the same kernels on the same buffers
+  // as any other step with the same number of products still folding (na)
+  // and the same side of the P12 / P22 ping-pong, once the layer's S11 and
+  // S12 are staged in the fold's X11 / X21 slots (which hold S_ref and S_trn
+  // only before the first and for the last step).  In the FP64 form each
+  // such step is captured once as a HIP graph and replayed: one launch
+  // instead of the hundreds (thousands for n > 1024) of a step's products
+  // and inverse.  RCWA_FOLD_GRAPHS=0 launches every step directly.
+  struct StepGraphs {
+    hipGraphExec_t ex[2] = {nullptr, nullptr};
+    int na[2] = {-1, -1};
+    ~StepGraphs() {
+      for (hipGraphExec_t e : ex)
+        if (e) (void)hipGraphExecDestroy(e);
+    }
+  } graphs;
+  const char* genv = getenv("RCWA_FOLD_GRAPHS");
+  const bool use_graphs = c.precision == RCWA_FP64 && !(genv && genv[0] == '0');
+  zc* const ping = fold.P12;  // P12 on side 0 of the ping-pong
+  auto layer_step = [&](int na, const zc* s11, const zc* s12) -> rcwa_status {
+    if (!use_graphs)  // layers: S21 = S12, S22 = S11
+      return fold.step(na, CMt(s11), CMt(s12), CMt(s12), CMt(s11), false);
+    CK(zcopy_batched(fs, n, n, CMt(s11), ZMat{fold.X11, n, n2}, na));
+    CK(zcopy_batched(fs, n, n, CMt(s12), ZMat{fold.X21, n, n2}, na));
+    const ZCMat x11 = CMt(fold.X11), x12 = CMt(fold.X21);
+    const int side = fold.P12 == ping ? 0 : 1;
+    if (graphs.na[side] != na) {
+      if (graphs.ex[side]) {
+        CKH(hipGraphExecDestroy(graphs.ex[side]));
+        graphs.ex[side] = nullptr;
+        graphs.na[side] = -1;
+      }
+      // the capture runs the step's host side (its ping-pong swap) and
+      // records its launches; the graph launch below runs them
+      hipGraph_t g = nullptr;
+      CKH(hipStreamBeginCapture(fs, hipStreamCaptureModeThreadLocal));
+      const rcwa_status st = fold.step(na, x11, x12, x12, x11, false);
+      const hipError_t ec = hipStreamEndCapture(fs, &g);
+      if (st != RCWA_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      CKH(ec);
+      hipGraphExec_t ex = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      CKH(ei);
+      graphs.ex[side] = ex;
+      graphs.na[side] = na;
+    } else {
+      // a replay does on the host what the captured step did
+      std::swap(fold.P12, fold.P12n);
+      std::swap(fold.P22, fold.P22n);
+    }
+    CKH(hipGraphLaunch(graphs.ex[side], fs));
+    return RCWA_OK;
   };
   int lf = 0;  // next fold step
   // ---- SMatrixLayer for every layer (main.cpp:151)
@@ -903,11 +957,9 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   auto queue_fold = [&](int upto) -> rcwa_status {
     if (lf >= Lmax || off[lf + 1] > upto) return RCWA_OK;
     if (rcwa_status e = fence(s, fs)) return e;
-    for (; lf < Lmax && off[lf + 1] <= upto; ++lf) {
-      const ZCMat x11 = CMt(S11l + (size_t)off[lf] * n2), x12 = CMt(S12l + (size_t)off[lf] * n2);
-      if (rcwa_status e = step(act[lf], x11, x12, x12, x11, false))  // S21 = S12, S22 = S11
+    for (; lf < Lmax && off[lf + 1] <= upto; ++lf)
+      if (rcwa_status e = layer_step(act[lf], S11l + (size_t)off[lf] * n2, S12l + (size_t)off[lf] * n2))
         return e;
-    }
     return RCWA_OK;
   };
   // With the fold on its own stream, each layer batch is finished (after its
