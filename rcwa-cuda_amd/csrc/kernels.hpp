@@ -31,6 +31,10 @@ void zgemm_force_shape(int shape);
 int zgemm_tuned_entries(int* out, int max_entries);
 int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
                       ZCMat D, ZMat C, int batch, int tri);
+// zgemm_batched for the products b with sel[b] != 0 (device flags); C of the
+// others is left as it was.
+int zgemm_batched_sel(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                      ZCMat D, ZMat C, int batch, const int* sel);
 
 // linalg.hip -----------------------------------------------------------------
 // In-place inverse of a batch of n x n matrices by blocked Gauss-Jordan
@@ -49,6 +53,11 @@ int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* i
 // block_inverse_ws_bytes(n, cap) bytes for a batch <= cap.
 size_t block_inverse_ws_bytes(int n, int cap);
 int block_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* info);
+
+// flag[b] = 1 when ||M_b||_1 ||F_b||_1 > kappa (column sums of |re| + |im|,
+// within a factor 2 of the complex 1-norms), else 0: with F = M^-1 an
+// estimate of M's 1-norm condition number.
+int cond1_flags(hipStream_t s, int n, ZCMat M, ZCMat F, double kappa, int* flag, int batch);
 
 // Elementwise helpers.
 int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag);

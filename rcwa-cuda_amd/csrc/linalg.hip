@@ -560,6 +560,51 @@ int block_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int
   return block_inverse_rec(s, n, M, batch, ws, rws, info);
 }
 
+namespace {
+
+// One workgroup of four waves per matrix: each wave sums columns of M and F
+// with its lanes along the rows, then the largest sums are reduced.
+__global__ __launch_bounds__(256) void cond1_kernel(int n, ZCMat Mm, ZCMat Fm, double kappa,
+                                                    int* flag) {
+  __shared__ double red[2][4];
+  const int b = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const zc* Mp = Mm.p + (long long)b * Mm.stride;
+  const zc* Fp = Fm.p + (long long)b * Fm.stride;
+  double mm = 0.0, mf = 0.0;
+  for (int j = wave; j < n; j += 4) {
+    double cm = 0.0, cf = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const zc a = Mp[i + (long long)j * Mm.ld], f = Fp[i + (long long)j * Fm.ld];
+      cm += fabs(a.x) + fabs(a.y);
+      cf += fabs(f.x) + fabs(f.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      cm += __shfl_xor(cm, o);
+      cf += __shfl_xor(cf, o);
+    }
+    mm = fmax(mm, cm);
+    mf = fmax(mf, cf);
+  }
+  if (lane == 0) {
+    red[0][wave] = mm;
+    red[1][wave] = mf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double a = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+    const double f = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
+    flag[b] = !(a * f <= kappa);  // a NaN estimate counts as ill-conditioned
+  }
+}
+
+}  // namespace
+
+int cond1_flags(hipStream_t s, int n, ZCMat M, ZCMat F, double kappa, int* flag, int batch) {
+  if (n <= 0 || batch <= 0) return 0;
+  hipLaunchKernelGGL(cond1_kernel, dim3(batch), dim3(256), 0, s, n, M, F, kappa, flag);
+  return (int)hipGetLastError();
+}
+
 int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag) {
   hipLaunchKernelGGL(identity_kernel, grid_for((long long)n * n, batch), dim3(256), 0, s, n, M,
                      diag);

@@ -455,6 +455,7 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
   z += 13 * (size_t)G * n2;                         // fold slots
   z += (size_t)G * 32 * N + (size_t)G * 4 * n;      // end diagonals, gathered columns
   size_t bytes = z * sizeof(zc) + T1 * (2 * sizeof(double) + 3 * sizeof(int)) + 64;
+  bytes += (size_t)std::max(G, 1) * sizeof(int) + 256;  // the fold's refinement flags
   bytes += layer_work_bytes(N, Bl) + block_inverse_ws_bytes((int)n, std::max(G, 1)) +
            eig_workspace_bytes((int)n, Bl) + 64 * 256;
   return bytes;
@@ -494,7 +495,11 @@ static size_t sweep_chunk_extra_bytes(int N, int L, int G, int M, int share, boo
 // explicit inverse is then much less accurate than a backward-stable solve
 // (the reference's LU solves, rcwa.cpp:602-606); the refined solve is as
 // accurate as one.  Level 1 refines the column solves (two n x 2 products),
-// level 2 also F X12 (two n^3 products).
+// level 2 also F X12 (two n^3 products).  In the FP64 form level 2 refines
+// only the products whose M has ||M||_1 ||F||_1 above kRefineKappa1: below
+// it the product with F is within about kappa sqrt(n) eps of the solve's
+// accuracy, 1e-12 relative at n = 882, and the two n^3 products are skipped.
+constexpr double kRefineKappa1 = 1e3;
 int Fold::gemm(int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D_, ZMat C_,
                int bt) {
   return on2 ? c->gemm2(M_, N_, K_, al, A_, B_, be, D_, C_, bt)
@@ -507,7 +512,8 @@ static rcwa_status fold_check(const Fold& f) {
   const zc* p[] = {f.P12, f.P22, f.P12n, f.P22n, f.Fm, f.Gm, f.Hm, f.Id, f.Mf, f.vu, f.X11, f.X21, f.va, f.vb};
   for (const zc* q : p)
     if (!q) return rcwa_set_error(RCWA_E_ARG, "fold work slot not set");
-  if (!f.colidx || !f.gj || !f.c || !f.fs) return rcwa_set_error(RCWA_E_ARG, "fold not set up");
+  if (!f.colidx || !f.gj || !f.c || !f.fs || !f.rflag)
+    return rcwa_set_error(RCWA_E_ARG, "fold not set up");
   return RCWA_OK;
 }
 
@@ -559,8 +565,14 @@ rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool 
   if (columns_only) return RCWA_OK;
   CK(gemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
   if (refine > 1) {
-    CK(gemm(n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na));
-    CK(gemm(n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na));
+    if (c->precision == RCWA_FP64) {
+      CK(cond1_flags(fs, n, CMt(Mf), CMt(Fm), kRefineKappa1, rflag, na));
+      CK(zgemm_batched_sel(fs, n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na, rflag));
+      CK(zgemm_batched_sel(fs, n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na, rflag));
+    } else {
+      CK(gemm(n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na));
+      CK(gemm(n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na));
+    }
   }
   CK(gemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
   CK(gemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
@@ -703,6 +715,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   double* dcond = (double*)c.take(T1 * sizeof(double));
   int* dnonfin = (int*)c.take(sizeof(int));
   int* dcolidx = (int*)c.take(4 * sizeof(int));
+  int* drflag = (int*)c.take((size_t)std::max(G, 1) * sizeof(int));
   void* gjf = c.take(block_inverse_ws_bytes(n, std::max(G, 1)));  // the fold's inverses (and the row inverses)
   LayerWork w;
   w.carve(c, N, Bl);
@@ -886,6 +899,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   fold.vb = dcols + (size_t)G * 2 * n;
   fold.gj = gjf;
   fold.info = info_sing;
+  fold.rflag = drflag;
   fold.colidx = dcolidx;
   if ((st = fold.start(dends, 32LL * N, G)) != RCWA_OK) return st;
   auto CMt = [&](const zc* p) { return ZCMat{p, n, n2}; };

@@ -112,6 +112,7 @@ struct Fold {
      *X21 = nullptr, *va = nullptr, *vb = nullptr;
   void* gj = nullptr;        // block_inverse_ws_bytes(n, G) bytes
   int* info = nullptr;       // singular-pivot flags
+  int* rflag = nullptr;      // G ints: which products refine F X12 (FP64 form)
   const int* colidx = nullptr;  // device {N/2, N/2 + N}
   int gemm(int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int batch);
   // P = S_ref for the first cnt products, from their end diagonals (32 N

@@ -216,6 +216,7 @@ size_t session_bytes(int N, int Nx, int Ny, int hx, int hy, int S, int B, int Bl
   bytes += 2 * (2 * (size_t)B * sizeof(int) + (size_t)B * sizeof(double) +
                 index_ints(S) * sizeof(int) + 2 * (size_t)S * sizeof(int));
   bytes += layer_work_bytes(N, Bl) + block_inverse_ws_bytes((int)n, S) + 64 * 256;
+  bytes += (size_t)S * sizeof(int) + 256;  // the fold's refinement flags
   return bytes;
 }
 
@@ -346,6 +347,7 @@ rcwa_status Engine::open_session(int hx, int hy, int Nx, int Ny) {
   f.gj = c_.take(block_inverse_ws_bytes(se.n, S));
   se.w.carve(c_, se.N, B);
   se.dcolidx = (int*)c_.take(4 * sizeof(int));
+  f.rflag = (int*)c_.take((size_t)S * sizeof(int));
   if (rcwa_status st = c_.arena_check()) return st;
   const int hcol[4] = {se.N / 2, se.N / 2 + se.N, 0, 0};
   ECK(hipMemcpy(se.dcolidx, hcol, sizeof(hcol), hipMemcpyHostToDevice));

@@ -67,7 +67,7 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 template <int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WN >= 2 ? 2 : 1))) void zgemm_kernel(
     int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta, ZCMat D, ZMat C, int tiles_n,
-    int tri) {
+    int tri, const int* sel) {
   constexpr int BM = 32 * WM, BN = 32 * WN, NT = 64 * WM * WN;
   constexpr int LA = BM * BK / NT, LB = BK * BN / NT;  // slab entries per thread
   __shared__ zc As[2][BK][BM + PADA];
@@ -85,6 +85,7 @@ __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WN
   const int tn = t % tiles_n;
   const int tm = t / tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
+  if (sel && sel[b] == 0) return;  // a product not selected: C keeps its contents
   if ((tri & ZGEMM_TRI_C) && m0 >= n0 + BN) return;  // uniform over the workgroup
   if (tri & ZGEMM_TRI_B) K = min(K, n0 + BN);
   const int kbeg = (tri & ZGEMM_TRI_A) ? min(m0, K) : 0;  // a multiple of BK
@@ -225,7 +226,7 @@ constexpr Shape kShapes[] = {{2, 2}, {1, 1}, {4, 1}, {2, 4}, {4, 2}};
 constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
 int launch_shape(int v, hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
-                 ZCMat D, ZMat C, int batch, int tri) {
+                 ZCMat D, ZMat C, int batch, int tri, const int* sel = nullptr) {
   const int bm = 32 * kShapes[v].wm, bn = 32 * kShapes[v].wn;
   const int tiles_m = (M + bm - 1) / bm, tiles_n = (N + bn - 1) / bn;
   const long long total = (long long)tiles_m * tiles_n * batch;
@@ -234,23 +235,23 @@ int launch_shape(int v, hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, Z
   switch (v) {
     case 1:
       hipLaunchKernelGGL((zgemm_kernel<1, 1>), grid, block, 0, s, M, N, K, alpha, A, B, beta, D, C,
-                         tiles_n, tri);
+                         tiles_n, tri, sel);
       break;
     case 2:
       hipLaunchKernelGGL((zgemm_kernel<4, 1>), grid, block, 0, s, M, N, K, alpha, A, B, beta, D, C,
-                         tiles_n, tri);
+                         tiles_n, tri, sel);
       break;
     case 3:
       hipLaunchKernelGGL((zgemm_kernel<2, 4>), grid, block, 0, s, M, N, K, alpha, A, B, beta, D, C,
-                         tiles_n, tri);
+                         tiles_n, tri, sel);
       break;
     case 4:
       hipLaunchKernelGGL((zgemm_kernel<4, 2>), grid, block, 0, s, M, N, K, alpha, A, B, beta, D, C,
-                         tiles_n, tri);
+                         tiles_n, tri, sel);
       break;
     default:
       hipLaunchKernelGGL((zgemm_kernel<2, 2>), grid, block, 0, s, M, N, K, alpha, A, B, beta, D, C,
-                         tiles_n, tri);
+                         tiles_n, tri, sel);
   }
   return (int)hipGetLastError();
 }
@@ -424,4 +425,11 @@ int zgemm_batched_tri(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCM
 int zgemm_batched(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
                   ZCMat D, ZMat C, int batch) {
   return zgemm_batched_tri(s, M, N, K, alpha, A, B, beta, D, C, batch, 0);
+}
+
+int zgemm_batched_sel(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCMat B, zc beta,
+                      ZCMat D, ZMat C, int batch, const int* sel) {
+  if (M <= 0 || N <= 0 || batch <= 0) return 0;
+  return launch_shape(shape_for(s, M, N, K, batch, 0), s, M, N, K, alpha, A, B, beta, D, C, batch,
+                      0, sel);
 }
