@@ -43,12 +43,12 @@ int zgemm_batched_sel(hipStream_t s, int M, int N, int K, zc alpha, ZCMat A, ZCM
 // exactly zero (the matrix is singular); it is not cleared.
 size_t gj_workspace_bytes(int n, int batch);
 int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* info);
-// In-place inverse for the fold's I - X11 P22: for n > 1024 in batches of at
-// most 8 (where Gauss-Jordan's n/4 panel steps on one workgroup per matrix
-// leave the chip idle) by 2x2-block recursion on the Schur complement, the
-// blocks of at most 1024 rows inverted by Gauss-Jordan with partial pivoting
-// inside the block and the rest as half-size ZGEMMs; otherwise
-// gj_inverse_batched.  No pivoting across blocks: for matrices whose leading
+// In-place inverse for the fold's I - X11 P22: for n > 512 in batches of at
+// most 64 (where Gauss-Jordan's n/16 .. n/4 panel steps on one workgroup per
+// matrix leave most of the chip idle) by 2x2-block recursion on the Schur
+// complement, the blocks of at most 512 rows inverted by Gauss-Jordan with
+// partial pivoting inside the block and the rest as half-size ZGEMMs;
+// otherwise gj_inverse_batched.  No pivoting across blocks: for matrices whose leading
 // blocks are well conditioned, as I - K with small K.  ws must hold
 // block_inverse_ws_bytes(n, cap) bytes for a batch <= cap.
 size_t block_inverse_ws_bytes(int n, int cap);

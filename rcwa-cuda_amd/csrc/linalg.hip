@@ -498,8 +498,8 @@ int gj_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int* i
 
 namespace {
 
-constexpr int BLK_INV_BASE = 1024;  // blocks this size or smaller: Gauss-Jordan
-constexpr int BLK_INV_MAXB = 8;     // larger batches keep Gauss-Jordan busy enough
+constexpr int BLK_INV_BASE = 512;  // blocks this size or smaller: Gauss-Jordan
+constexpr int BLK_INV_MAXB = 64;   // larger batches keep Gauss-Jordan busy enough
 
 bool use_block_inverse(int n, int batch) { return n > BLK_INV_BASE && batch <= BLK_INV_MAXB; }
 

@@ -551,7 +551,12 @@ rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool 
   const int refine = c->refine;
   CK(gemm(n, n, n, mone, x11, CMt(P22), one, ident, Mt(Fm), na));
   if (refine > 0) CK(zcopy_batched(fs, n, n, CMt(Fm), Mt(Mf), na));
-  CK(block_inverse_batched(fs, n, Mt(Fm), na, gj, info));
+  // RCWA_FOLD_INVERSE=gj: Gauss-Jordan for every size and batch (tests)
+  const char* fi = getenv("RCWA_FOLD_INVERSE");
+  if (fi && fi[0] == 'g')
+    CK(gj_inverse_batched(fs, n, Mt(Fm), na, gj, info));
+  else
+    CK(block_inverse_batched(fs, n, Mt(Fm), na, gj, info));
   // the columns, with the old P12 and P22
   CK(gemm(n, 2, n, one, x11, CMv(vb), zero, none, Mv(vu), na));
   CK(gemm(n, 2, n, one, CMt(Fm), CMv(vu), zero, none, Mv(vw), na));
