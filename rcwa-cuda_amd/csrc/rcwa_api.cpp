@@ -912,20 +912,28 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   // as any other step with the same number of products still folding (na)
   // and the same side of the P12 / P22 ping-pong, once the layer's S11 and
   // S12 are staged in the fold's X11 / X21 slots (which hold S_ref and S_trn
-  // only before the first and for the last step).  In the FP64 form each
-  // such step is captured once as a HIP graph and replayed: one launch
-  // instead of the hundreds (thousands for n > 1024) of a step's products
-  // and inverse.  RCWA_FOLD_GRAPHS=0 launches every step directly.
+  // only before the first and for the last step).  With RCWA_FOLD_GRAPHS=1
+  // (FP64 form) each such step is captured once as a HIP graph and replayed:
+  // one launch instead of the hundreds (thousands for n > 1024) of a step's
+  // products and inverse.  Measured no faster than direct launches, so off
+  // by default.  A graph is destroyed only once the fold stream has drained.
   struct StepGraphs {
-    hipGraphExec_t ex[2] = {nullptr, nullptr};
+    hipStream_t fs = nullptr;
+    std::vector<hipGraphExec_t> ex{nullptr, nullptr}, retired;
     int na[2] = {-1, -1};
     ~StepGraphs() {
+      bool any = false;
+      for (hipGraphExec_t e : ex) any = any || e;
+      if (!any && retired.empty()) return;
+      (void)hipStreamSynchronize(fs);
       for (hipGraphExec_t e : ex)
         if (e) (void)hipGraphExecDestroy(e);
+      for (hipGraphExec_t e : retired) (void)hipGraphExecDestroy(e);
     }
   } graphs;
+  graphs.fs = fs;
   const char* genv = getenv("RCWA_FOLD_GRAPHS");
-  const bool use_graphs = c.precision == RCWA_FP64 && !(genv && genv[0] == '0');
+  const bool use_graphs = c.precision == RCWA_FP64 && genv && genv[0] == '1';
   zc* const ping = fold.P12;  // P12 on side 0 of the ping-pong
   auto layer_step = [&](int na, const zc* s11, const zc* s12) -> rcwa_status {
     if (!use_graphs)  // layers: S21 = S12, S22 = S11
@@ -935,8 +943,8 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     const ZCMat x11 = CMt(fold.X11), x12 = CMt(fold.X21);
     const int side = fold.P12 == ping ? 0 : 1;
     if (graphs.na[side] != na) {
-      if (graphs.ex[side]) {
-        CKH(hipGraphExecDestroy(graphs.ex[side]));
+      if (graphs.ex[side]) {  // may still run: destroyed with the others at the end
+        graphs.retired.push_back(graphs.ex[side]);
         graphs.ex[side] = nullptr;
         graphs.na[side] = -1;
       }

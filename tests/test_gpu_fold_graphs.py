@@ -1,5 +1,5 @@
-"""Fold steps replayed as HIP graphs (the FP64 form's default) against the
-same steps launched directly (RCWA_FOLD_GRAPHS=0): bit for bit, on chunks
+"""Fold steps replayed as HIP graphs (RCWA_FOLD_GRAPHS=1, FP64 form) against
+the same steps launched directly (the default): bit for bit, on chunks
 whose structures have different layer counts (so that the number of
 products still folding changes and steps are captured anew), over several
 solves on one context with the layer stage's own work running between the
@@ -28,13 +28,14 @@ def test_fold_graphs_equal_direct_launches(monkeypatch, h, grid, layers):
     probs = _chunk(100 + h, h, grid, layers)
     ctx = G.Context(0)
     try:
-        monkeypatch.delenv("RCWA_FOLD_GRAPHS", raising=False)
+        monkeypatch.setenv("RCWA_FOLD_GRAPHS", "1")
         first = ctx.solve(probs)
         again = ctx.solve(probs)
-        monkeypatch.setenv("RCWA_FOLD_GRAPHS", "0")
-        direct = ctx.solve(probs)
         monkeypatch.delenv("RCWA_FOLD_GRAPHS")
+        direct = ctx.solve(probs)
+        monkeypatch.setenv("RCWA_FOLD_GRAPHS", "1")
         third = ctx.solve(probs)
+        monkeypatch.delenv("RCWA_FOLD_GRAPHS")
     finally:
         ctx.close()
     for i in range(len(probs)):
