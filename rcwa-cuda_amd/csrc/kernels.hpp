@@ -56,8 +56,11 @@ int block_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int
 
 // flag[b] = 1 when ||M_b||_1 ||F_b||_1 > kappa (column sums of |re| + |im|,
 // within a factor 2 of the complex 1-norms), else 0: with F = M^-1 an
-// estimate of M's 1-norm condition number.
-int cond1_flags(hipStream_t s, int n, ZCMat M, ZCMat F, double kappa, int* flag, int batch);
+// estimate of M's 1-norm condition number.  scratch: device memory of
+// cond1_scratch_bytes(n, batch) bytes.
+size_t cond1_scratch_bytes(int n, int batch);
+int cond1_flags(hipStream_t s, int n, ZCMat M, ZCMat F, double kappa, int* flag, int batch,
+                double* scratch);
 
 // Elementwise helpers.
 int zset_identity(hipStream_t s, int n, ZMat M, int batch, zc diag);

@@ -562,16 +562,19 @@ int block_inverse_batched(hipStream_t s, int n, ZMat M, int batch, void* ws, int
 
 namespace {
 
-// One workgroup of four waves per matrix: each wave sums columns of M and F
-// with its lanes along the rows, then the largest sums are reduced.
-__global__ __launch_bounds__(256) void cond1_kernel(int n, ZCMat Mm, ZCMat Fm, double kappa,
-                                                    int* flag) {
+// Column sums of |re| + |im| of M and F, COND_COLS columns per workgroup of
+// four waves (a wave per column at a time, lanes along the rows); each
+// workgroup writes the largest sums of its columns to part[b][chunk][2].
+constexpr int COND_COLS = 16;
+__global__ __launch_bounds__(256) void cond1_part_kernel(int n, ZCMat Mm, ZCMat Fm, double* part) {
   __shared__ double red[2][4];
-  const int b = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b = blockIdx.y, chunk = blockIdx.x, nch = gridDim.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const zc* Mp = Mm.p + (long long)b * Mm.stride;
   const zc* Fp = Fm.p + (long long)b * Fm.stride;
   double mm = 0.0, mf = 0.0;
-  for (int j = wave; j < n; j += 4) {
+  const int j1 = min(n, (chunk + 1) * COND_COLS);
+  for (int j = chunk * COND_COLS + wave; j < j1; j += 4) {
     double cm = 0.0, cf = 0.0;
     for (int i = lane; i < n; i += 64) {
       const zc a = Mp[i + (long long)j * Mm.ld], f = Fp[i + (long long)j * Fm.ld];
@@ -591,17 +594,42 @@ __global__ __launch_bounds__(256) void cond1_kernel(int n, ZCMat Mm, ZCMat Fm, d
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const double a = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
-    const double f = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
-    flag[b] = !(a * f <= kappa);  // a NaN estimate counts as ill-conditioned
+    double* o = part + ((long long)b * nch + chunk) * 2;
+    o[0] = fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3]));
+    o[1] = fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3]));
   }
+}
+
+// One wave per matrix: the largest column sums over the chunks, then the flag.
+__global__ __launch_bounds__(64) void cond1_flag_kernel(int nch, const double* part, double kappa,
+                                                        int* flag) {
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const double* p = part + (long long)b * nch * 2;
+  double a = 0.0, f = 0.0;
+  for (int c = lane; c < nch; c += 64) {
+    a = fmax(a, p[2 * c]);
+    f = fmax(f, p[2 * c + 1]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = fmax(a, __shfl_xor(a, o));
+    f = fmax(f, __shfl_xor(f, o));
+  }
+  if (lane == 0) flag[b] = !(a * f <= kappa);  // a NaN estimate counts as ill-conditioned
 }
 
 }  // namespace
 
-int cond1_flags(hipStream_t s, int n, ZCMat M, ZCMat F, double kappa, int* flag, int batch) {
+size_t cond1_scratch_bytes(int n, int batch) {
+  return (size_t)batch * ((n + COND_COLS - 1) / COND_COLS) * 2 * sizeof(double);
+}
+
+int cond1_flags(hipStream_t s, int n, ZCMat M, ZCMat F, double kappa, int* flag, int batch,
+                double* scratch) {
   if (n <= 0 || batch <= 0) return 0;
-  hipLaunchKernelGGL(cond1_kernel, dim3(batch), dim3(256), 0, s, n, M, F, kappa, flag);
+  const int nch = (n + COND_COLS - 1) / COND_COLS;
+  hipLaunchKernelGGL(cond1_part_kernel, dim3(nch, batch), dim3(256), 0, s, n, M, F, scratch);
+  if (int e = (int)hipGetLastError()) return e;
+  hipLaunchKernelGGL(cond1_flag_kernel, dim3(batch), dim3(64), 0, s, nch, scratch, kappa, flag);
   return (int)hipGetLastError();
 }
 

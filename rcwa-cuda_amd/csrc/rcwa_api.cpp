@@ -571,7 +571,9 @@ rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool 
   CK(gemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
   if (refine > 1) {
     if (c->precision == RCWA_FP64) {
-      CK(cond1_flags(fs, n, CMt(Mf), CMt(Fm), kRefineKappa1, rflag, na));
+      // Hm is free until the first refinement product: it holds the
+      // estimate's partial column sums (far fewer than its G n^2 entries)
+      CK(cond1_flags(fs, n, CMt(Mf), CMt(Fm), kRefineKappa1, rflag, na, (double*)Hm));
       CK(zgemm_batched_sel(fs, n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na, rflag));
       CK(zgemm_batched_sel(fs, n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na, rflag));
     } else {
