@@ -495,10 +495,10 @@ static size_t sweep_chunk_extra_bytes(int N, int L, int G, int M, int share, boo
 // explicit inverse is then much less accurate than a backward-stable solve
 // (the reference's LU solves, rcwa.cpp:602-606); the refined solve is as
 // accurate as one.  Level 1 refines the column solves (two n x 2 products),
-// level 2 also F X12 (two n^3 products).  In the FP64 form level 2 refines
-// only the products whose M has ||M||_1 ||F||_1 above kRefineKappa1: below
-// it the product with F is within about kappa sqrt(n) eps of the solve's
-// accuracy, 1e-12 relative at n = 882, and the two n^3 products are skipped.
+// level 2 also F X12 (two n^3 products).  Level 2 refines only the
+// products whose M has ||M||_1 ||F||_1 above kRefineKappa1: below it the
+// product with F is within about kappa sqrt(n) eps of the solve's accuracy,
+// 1e-12 relative at n = 882, and the two n^3 products are skipped.
 constexpr double kRefineKappa1 = 1e3;
 int Fold::gemm(int M_, int N_, int K_, zc al, ZCMat A_, ZCMat B_, zc be, ZCMat D_, ZMat C_,
                int bt) {
@@ -570,16 +570,12 @@ rcwa_status Fold::step(int na, ZCMat x11, ZCMat x12, ZCMat x21, ZCMat x22, bool 
   if (columns_only) return RCWA_OK;
   CK(gemm(n, n, n, one, CMt(Fm), x12, zero, none, Mt(Gm), na));
   if (refine > 1) {
-    if (c->precision == RCWA_FP64) {
-      // Hm is free until the first refinement product: it holds the
-      // estimate's partial column sums (far fewer than its G n^2 entries)
-      CK(cond1_flags(fs, n, CMt(Mf), CMt(Fm), kRefineKappa1, rflag, na, (double*)Hm));
-      CK(zgemm_batched_sel(fs, n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na, rflag));
-      CK(zgemm_batched_sel(fs, n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na, rflag));
-    } else {
-      CK(gemm(n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na));
-      CK(gemm(n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na));
-    }
+    // Hm is free until the first refinement product: it holds the
+    // estimate's partial column sums (far fewer than its G n^2 entries).
+    // The refinement products are FP64 in both precision forms.
+    CK(cond1_flags(fs, n, CMt(Mf), CMt(Fm), kRefineKappa1, rflag, na, (double*)Hm));
+    CK(zgemm_batched_sel(fs, n, n, n, mone, CMt(Mf), CMt(Gm), one, x12, Mt(Hm), na, rflag));
+    CK(zgemm_batched_sel(fs, n, n, n, one, CMt(Fm), CMt(Hm), one, CMt(Gm), Mt(Gm), na, rflag));
   }
   CK(gemm(n, n, n, one, CMt(P12), CMt(Gm), zero, none, Mt(P12n), na));
   CK(gemm(n, n, n, one, CMt(P22), CMt(Gm), zero, none, Mt(Hm), na));
