@@ -42,6 +42,10 @@ __device__ __forceinline__ zc wave_sum(zc v) {
 }
 
 // Workspace per matrix: V (n x NB), Y (n x NB), T (NB x NB), column ld = n.
+// RN rows of the panel's product per thread (n <= RN * PT), CB columns per
+// iteration of it (their loads issued together; the sums run in column order
+// whatever CB is).
+template <int RN, int CB>
 __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, ZMat Am, zc* Vw,
                                                         zc* Yw, zc* Tw) {
   extern __shared__ zc sm[];
@@ -134,22 +138,22 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
       acc = wave_sum(acc);
       if (lane == 0) zq[q] = acc;
     }
-    zc yacc[(4096 + PT - 1) / PT];
+    zc yacc[RN];
 #pragma unroll
-    for (int r = 0; r < (4096 + PT - 1) / PT; ++r) yacc[r] = zmake(0, 0);
-    // eight columns per iteration, their loads issued together
+    for (int r = 0; r < RN; ++r) yacc[r] = zmake(0, 0);
+    // CB columns per iteration, their loads issued together
     int c = j + 1;
-    for (; c + 7 < n; c += 8) {
+    for (; c + CB - 1 < n; c += CB) {
       const zc* col = A + (long long)c * lda;
 #pragma unroll
-      for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
+      for (int r = 0; r < RN; ++r) {
         const int i = p + 1 + tid + r * PT;
         if (i < n) {
-          zc a[8];
+          zc a[CB];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) a[q] = col[i + q * lda];
+          for (int q = 0; q < CB; ++q) a[q] = col[i + q * lda];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) yacc[r] = zfma(a[q], bcol[c + q], yacc[r]);
+          for (int q = 0; q < CB; ++q) yacc[r] = zfma(a[q], bcol[c + q], yacc[r]);
         }
       }
     }
@@ -157,14 +161,14 @@ __global__ __launch_bounds__(PT) void hess_panel_kernel(int n, int p, int nbe, Z
       const zc vc = bcol[c];
       const zc* col = A + (long long)c * lda;
 #pragma unroll
-      for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
+      for (int r = 0; r < RN; ++r) {
         const int i = p + 1 + tid + r * PT;
         if (i < n) yacc[r] = zfma(col[i], vc, yacc[r]);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < (4096 + PT - 1) / PT; ++r) {
+    for (int r = 0; r < RN; ++r) {
       const int i = p + 1 + tid + r * PT;
       if (i < n) {
         zc y = yacc[r];
@@ -412,8 +416,14 @@ int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, 
                            ZCMat{A.p, A.ld, A.stride}, V, Y, ZQ);
       }
     } else {
-      hipLaunchKernelGGL(hess_panel_kernel, dim3(batch), dim3(PT), (size_t)n * sizeof(zc), s, n,
-                         p, nbe, A, V, Y, T);
+      // one product row per thread up to n = PT, else up to four (sized
+      // registers for the product; eight columns in flight either way)
+      if (n <= PT)
+        hipLaunchKernelGGL((hess_panel_kernel<1, 8>), dim3(batch), dim3(PT), (size_t)n * sizeof(zc),
+                           s, n, p, nbe, A, V, Y, T);
+      else
+        hipLaunchKernelGGL((hess_panel_kernel<(4096 + PT - 1) / PT, 8>), dim3(batch), dim3(PT),
+                           (size_t)n * sizeof(zc), s, n, p, nbe, A, V, Y, T);
     }
     const int c0 = p + nbe;                   // first column right of the panel
     const int m = n - c0;                     // trailing columns
