@@ -113,7 +113,9 @@ typedef struct {
   double flops_algorithmic; /* 2384 N^3 per layer + 1621 N^3 per S-matrix */
   int n_smatrices, n_layers;
   /* eigensolver phases, filled only when rcwa_ctx_set_timing(ctx, 1): the
-   * stream is synchronised after each layer batch to read the events */
+   * stream is synchronised after each layer batch to read the events.  A
+   * small layer batch run as two staggered halves on two streams counts
+   * each half, so these sums can exceed the wall time they cover. */
   double ms_eig_hess, ms_eig_schur, ms_eig_vec;
   int n_eig_launches, n_eig_matrices;
 } rcwa_stats;

@@ -2190,7 +2190,7 @@ size_t schur_ws_bytes(int n, int batch) {
 }
 
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
-                  int maxit_per_row, void* ws, long long log_cap) {
+                  int maxit_per_row, void* ws, long long log_cap, int phase) {
   if (n <= 0 || batch <= 0) return 0;
   // The default is the multishift iteration with aggressive early deflation
   // (schur_ms_kernel).  RCWA_QR_SWEEP (read on every call) selects another
@@ -2227,17 +2227,22 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
     // a smaller limit (at least one sweep's worth) only makes the sweep kernel
     // stop for its log to be applied more often
     log.limit = log_cap > 0 ? std::max<long long>(std::min(log_cap, cap_full), n) : cap_full;
-    hipLaunchKernelGGL(qr_state_init_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, n,
-                       batch, log.state, log.pending);
-    int e = (int)hipGetLastError();
-    if (e) return e;
+    int e = 0;
+    if (phase != 2) {
+      hipLaunchKernelGGL(qr_state_init_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, n,
+                         batch, log.state, log.pending);
+      if ((e = (int)hipGetLastError())) return e;
+    }
     // Each round: sweeps until done or log full, then the log applied.  A
     // round makes progress on every pending matrix (at least one sweep), and
-    // the sweep limit bounds the sweeps, so the loop ends.
+    // the sweep limit bounds the sweeps, so the loop ends.  Phase 1 queues
+    // the first round and returns; phase 2 starts at its read-back.
     const bool huge = (long long)T.ld * n * 16 >= (1LL << 31);
     const bool ms = !huge && mode != 'o' && mode != 's' && mode != 'p';
-    for (;;) {
-      if (mode == 'o' || huge)
+    for (bool queued = phase == 2;; queued = false) {
+      if (queued) {
+        // the first round, queued by phase 1
+      } else if (mode == 'o' || huge)
         hipLaunchKernelGGL((schur_defer_kernel<12, 256>), dim3(batch), dim3(256), 0, s, n, T, Z,
                            info, maxit_per_row, log);
       else if (mode == 's')
@@ -2253,7 +2258,9 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
       // the log reaches the Schur vectors only: the sweeps keep the active
       // block of T up to date, and eig_batched forms the whole Schur form
       // afterwards as Z^H A Z
-      if (ms)
+      if (queued)
+        ;
+      else if (ms)
         hipLaunchKernelGGL((qr_apply_ms_kernel<APPLY_G, APPLY_TC, APPLY_NT, MS_NW>),
                            dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, Z,
                            log, blog);
@@ -2262,6 +2269,7 @@ int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info,
                            dim3((n + APPLY_NT - 1) / APPLY_NT, batch), dim3(APPLY_NT), 0, s, n, Z,
                            log);
       if ((e = (int)hipGetLastError())) return e;
+      if (phase == 1) return 0;
       int pending = 0;
       if ((e = (int)hipMemcpyAsync(&pending, log.pending, sizeof(int), hipMemcpyDeviceToHost, s)))
         return e;
@@ -2311,16 +2319,25 @@ size_t eig_workspace_bytes(int n, int batch) {
          schur_ws_bytes(n, batch) + 1024;
 }
 
+static int eig_finish(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, zc* Zp, zc* Xp,
+                      zc* Ap, void* hws, hipEvent_t* marks, bool inverse);
+
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
-                int maxit_per_row, hipEvent_t* marks, long long log_cap, bool inverse) {
+                int maxit_per_row, hipEvent_t* marks, long long log_cap, bool inverse,
+                int phase) {
   if (n <= 0 || batch <= 0) return 0;
   zc* Zp = (zc*)ws;
   zc* Xp = Zp + (size_t)batch * n * n;
   zc* Ap = Xp + (size_t)batch * n * n;  // a copy of A, later Z^H
   const long long nn = (long long)n * n;
-  ZMat Z{Zp, n, nn}, X{Xp, n, nn};
+  ZMat Z{Zp, n, nn};
   void* hws = (void*)(((uintptr_t)(Ap + (size_t)batch * n * n) + 255) & ~(uintptr_t)255);
+  void* sws = (void*)(((uintptr_t)hws + hess_blocked_ws_bytes(n, batch) + 255) & ~(uintptr_t)255);
   int e;
+  if (phase == 2) {
+    if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws, log_cap, 2))) return e;
+    return eig_finish(s, n, A, w, W, batch, Zp, Xp, Ap, hws, marks, inverse);
+  }
   if (marks) (void)hipEventRecord(marks[0], s);
   if ((e = zcopy_batched(s, n, n, ZCMat{A.p, A.ld, A.stride}, ZMat{Ap, n, nn}, batch))) return e;
   // RCWA_HESS=unblocked selects the one-workgroup unblocked reduction.
@@ -2334,8 +2351,18 @@ int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws
     if ((e = hessenberg_blocked_batched(s, n, A, Z, batch, hws))) return e;
   }
   if (marks) (void)hipEventRecord(marks[1], s);
-  void* sws = (void*)(((uintptr_t)hws + hess_blocked_ws_bytes(n, batch) + 255) & ~(uintptr_t)255);
-  if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws, log_cap))) return e;
+  if ((e = schur_batched(s, n, A, Z, batch, info, maxit_per_row, sws, log_cap, phase))) return e;
+  if (phase == 1) return 0;
+  return eig_finish(s, n, A, w, W, batch, Zp, Xp, Ap, hws, marks, inverse);
+}
+
+// After the QR iteration: the Schur form, its eigenvectors and W or W^-1
+// (Zp, Xp, Ap and hws: eig_batched's workspace parts).
+static int eig_finish(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, zc* Zp, zc* Xp,
+                      zc* Ap, void* hws, hipEvent_t* marks, bool inverse) {
+  const long long nn = (long long)n * n;
+  ZMat Z{Zp, n, nn}, X{Xp, n, nn};
+  int e;
   // The Schur form T = Z^H A Z on the matrix cores (the sweeps left the
   // entries of T outside each active block stale): X <- A Z, Ap <- Z^H,
   // A <- Z^H (A Z).

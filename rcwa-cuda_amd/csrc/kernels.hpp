@@ -95,9 +95,14 @@ size_t eig_workspace_bytes(int n, int batch);
 // inverse: W receives the inverse of the unit-norm eigenvector matrix
 // instead (formed from the Schur vectors and the triangular eigenvectors of
 // the Schur form, without a general inverse).
+// phase: 0 the whole decomposition; 1 its first part, which returns once the
+// QR iteration's first round is queued (nothing waits on the stream); 2 the
+// rest, called with the same arguments after phase 1 (it first waits for that
+// round).  Phases 1 and 2 let a caller queue work on other streams while the
+// QR iteration runs.
 int eig_batched(hipStream_t s, int n, ZMat A, zc* w, ZMat W, int batch, void* ws, int* info,
                 int maxit_per_row, hipEvent_t* marks = nullptr, long long log_cap = 0,
-                bool inverse = false);
+                bool inverse = false, int phase = 0);
 // The three phases, exposed for tests.
 int hessenberg_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch);
 // Blocked (compact-WY panels + ZGEMM updates); ws holds hess_blocked_ws_bytes(n, batch).
@@ -105,10 +110,11 @@ size_t hess_blocked_ws_bytes(int n, int batch);
 int hessenberg_blocked_batched(hipStream_t s, int n, ZMat A, ZMat Q, int batch, void* ws);
 // ws (schur_ws_bytes, required) holds the rotation and block logs: the sweeps
 // run on the active block and log their rotations, which are then applied to
-// the Schur vectors Z.
+// the Schur vectors Z.  phase as for eig_batched: 1 queues the first round
+// only, 2 waits for it and runs any further rounds, 0 both.
 size_t schur_ws_bytes(int n, int batch);
 int schur_batched(hipStream_t s, int n, ZMat T, ZMat Z, int batch, int* info, int maxit_per_row,
-                  void* ws, long long log_cap = 0);
+                  void* ws, long long log_cap = 0, int phase = 0);
 // unit: T has a unit diagonal; X = T^-1 (no shift).
 int trevc_batched(hipStream_t s, int n, ZCMat T, ZMat X, int batch, bool unit = false);
 

@@ -68,6 +68,7 @@ rcwa_status Ctx::init(int dev, size_t bud) {
   CKH(hipSetDevice(dev));
   CKH(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   CKH(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+  CKH(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
   size_t fr = 0, tot = 0;
   CKH(hipMemGetInfo(&fr, &tot));
   budget = bud ? bud : (size_t)(0.8 * (double)fr);
@@ -88,6 +89,8 @@ void Ctx::release() {
   arena_cap = 0;
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (stream3) (void)hipStreamDestroy(stream3);
+  stream3 = nullptr;
   if (stream2) (void)hipStreamDestroy(stream2);
   stream2 = nullptr;
   if (stream) (void)hipStreamDestroy(stream);
@@ -104,6 +107,7 @@ rcwa_status Ctx::reserve(size_t bytes) {
     CKH(hipSetDevice(device));
     CKH(hipStreamSynchronize(stream));
     if (stream2) CKH(hipStreamSynchronize(stream2));
+    if (stream3) CKH(hipStreamSynchronize(stream3));
     if (arena) CKH(hipFree(arena));
     arena = nullptr;
     arena_cap = 0;
@@ -122,6 +126,7 @@ rcwa_status Ctx::set_precision(rcwa_precision p) {
     CKH(hipSetDevice(device));
     CKH(hipStreamSynchronize(stream));
     if (stream2) CKH(hipStreamSynchronize(stream2));
+    if (stream3) CKH(hipStreamSynchronize(stream3));
     if (split_ws) CKH(hipFree(split_ws));
     if (split_ws2) CKH(hipFree(split_ws2));
     split_ws = split_ws2 = nullptr;
@@ -185,19 +190,32 @@ size_t layer_work_bytes(int N, int Bl) {
          10 * 256 + 4096;
 }
 
+static rcwa_status layer_stage_finish(Ctx& c, int N, int Bl, const zc* v0, const double* k0t,
+                                      zc* lam2, zc* lam, zc* ilam, zc* X, ZMat S11, ZMat S12,
+                                      LayerWork& w, int* info_sing, double* cond, int parts,
+                                      const std::function<rcwa_status(int)>& on_part,
+                                      hipEvent_t* marks);
+
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
                         ZCMat erci_pre, ZCMat urci_pre, double* cond, int parts,
-                        const std::function<rcwa_status(int)>& on_part) {
+                        const std::function<rcwa_status(int)>& on_part, int phase, int ev_group) {
   const int n = 2 * N;
   const long long n2 = (long long)n * n, N2 = (long long)N * N;
   hipStream_t s = c.stream;
   auto M = [&](zc* p) { return ZMat{p, n, n2}; };
   auto CM = [&](const zc* p) { return ZCMat{p, n, n2}; };
-  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const zc one = zmake(1, 0), zero = zmake(0, 0);
   const ZCMat none{nullptr, 0, 0};
-  zc *b0 = w.b[0], *b1 = w.b[1], *b2 = w.b[2], *b3 = w.b[3], *b4 = w.b[4], *b5 = w.b[5];
+  zc *b0 = w.b[0], *b1 = w.b[1], *b2 = w.b[2], *b3 = w.b[3];
+  hipEvent_t* marks = c.ev + 8 + 4 * ev_group;
+  if (phase == 2) {
+    CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
+                   marks, c.qr_log, true,2));
+    return layer_stage_finish(c, N, Bl, v0, k0t, lam2, lam, ilam, X, S11, S12, w, info_sing, cond,
+                              parts, on_part, marks);
+  }
   // [[eps]]^-1 and [[mu]]^-1 (rcwa.cpp:267-281: two LU solves each), unless
   // the caller formed them
   ZCMat erci{w.erci, N, N2}, urci{w.urci, N, N2};
@@ -224,13 +242,32 @@ rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* K
   // b3 <- W^-1 of the unit-norm eigenvectors (formed inside the eigensolver
   // from the Schur vectors and the triangular eigenvectors of the Schur form)
   CK(eig_batched(s, n, M(b2), lam2, M(b3), Bl, w.eig, info_eig, c.eig_maxit,
-                 c.time_eig ? c.ev + 8 : nullptr, c.qr_log, true));
+                 marks, c.qr_log, true,phase));
+  if (phase == 1) return RCWA_OK;
+  return layer_stage_finish(c, N, Bl, v0, k0t, lam2, lam, ilam, X, S11, S12, w, info_sing, cond,
+                            parts, on_part, marks);
+}
+
+// The layer stage after its eigensolver: from the eigenvectors' inverse W^-1
+// (b3) and the eigenvalues lambda^2 to S11, S12.
+static rcwa_status layer_stage_finish(Ctx& c, int N, int Bl, const zc* v0, const double* k0t,
+                                      zc* lam2, zc* lam, zc* ilam, zc* X, ZMat S11, ZMat S12,
+                                      LayerWork& w, int* info_sing, double* cond, int parts,
+                                      const std::function<rcwa_status(int)>& on_part,
+                                      hipEvent_t* marks) {
+  const int n = 2 * N;
+  const long long n2 = (long long)n * n;
+  hipStream_t s = c.stream;
+  auto CM = [&](const zc* p) { return ZCMat{p, n, n2}; };
+  const zc one = zmake(1, 0), mone = zmake(-1, 0), zero = zmake(0, 0);
+  const ZCMat none{nullptr, 0, 0};
+  zc *b0 = w.b[0], *b1 = w.b[1], *b2 = w.b[2], *b3 = w.b[3], *b4 = w.b[4], *b5 = w.b[5];
   if (c.time_eig) {
     float th = 0, ts = 0, tv = 0;
-    CKH(hipEventSynchronize(c.ev[11]));
-    CKH(hipEventElapsedTime(&th, c.ev[8], c.ev[9]));
-    CKH(hipEventElapsedTime(&ts, c.ev[9], c.ev[10]));
-    CKH(hipEventElapsedTime(&tv, c.ev[10], c.ev[11]));
+    CKH(hipEventSynchronize(marks[3]));
+    CKH(hipEventElapsedTime(&th, marks[0], marks[1]));
+    CKH(hipEventElapsedTime(&ts, marks[1], marks[2]));
+    CKH(hipEventElapsedTime(&tv, marks[2], marks[3]));
     c.stats.ms_eig += th + ts + tv;
     c.stats.ms_eig_hess += th;
     c.stats.ms_eig_schur += ts;
@@ -658,7 +695,32 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     const int nb = (TL + 2 * c.cus - 1) / (2 * c.cus);
     Bl = (TL + nb - 1) / nb;
   }
-  rcwa_status st = c.reserve(chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra);
+  size_t need = chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra;
+  // The QR iteration runs one workgroup per matrix: a layer batch of at most
+  // CUs / 2 matrices (n >= 256) leaves most of the chip idle while it runs,
+  // so such a batch, when it is the only one, is formed as two staggered
+  // halves on two streams (FP64 form; the split form's products share one
+  // slice workspace).  RCWA_LAYER_HALVES=0 turns this off, =1 on for any
+  // single batch of two or more matrices.
+  bool halves = c.stream2 && c.stream3 && c.precision == RCWA_FP64 && Bl >= TL && TL >= 2;
+  if (const char* he = getenv("RCWA_LAYER_HALVES"))
+    halves = halves && he[0] == '1';
+  else
+    halves = halves && 2 * N >= 256 && TL <= c.cus / 2;
+  // the first half (a first half of 30 or 40 % measured the same as 50 % on
+  // the 41 x 41-harmonic, 50-layer stack)
+  auto half1 = [](int b) { return (b + 1) / 2; };
+  if (halves) {
+    const size_t one = layer_work_bytes(N, Bl),
+                 two = layer_work_bytes(N, half1(TL)) + layer_work_bytes(N, TL - half1(TL));
+    if (two > one) {
+      if (need + (two - one) <= c.budget)
+        need += two - one;
+      else
+        halves = false;
+    }
+  }
+  rcwa_status st = c.reserve(need);
   if (st) return st;
 
   // ---- host precompute (O(N) per job)
@@ -720,8 +782,13 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   int* dcolidx = (int*)c.take(4 * sizeof(int));
   int* drflag = (int*)c.take((size_t)std::max(G, 1) * sizeof(int));
   void* gjf = c.take(block_inverse_ws_bytes(n, std::max(G, 1)));  // the fold's inverses (and the row inverses)
-  LayerWork w;
-  w.carve(c, N, Bl);
+  LayerWork w, w2;
+  if (halves) {
+    w.carve(c, N, half1(TL));
+    w2.carve(c, N, TL - half1(TL));
+  } else {
+    w.carve(c, N, Bl);
+  }
   zc *dcoef_e = nullptr, *dcoef_u = nullptr, *erci_all = nullptr, *urci_all = nullptr;
   zc* uinv = nullptr;
   int* dgmap = nullptr;
@@ -978,10 +1045,11 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   // batch)
   const bool progress = getenv("RCWA_PROGRESS") != nullptr;
   // fold steps whose layers are among the first `upto` layer matrices, queued
-  // on the fold stream behind the layer stage's work so far
-  auto queue_fold = [&](int upto) -> rcwa_status {
+  // on the fold stream behind the work queued so far on `from`, the stream
+  // that formed them
+  auto queue_fold = [&](int upto, hipStream_t from) -> rcwa_status {
     if (lf >= Lmax || off[lf + 1] > upto) return RCWA_OK;
-    if (rcwa_status e = fence(s, fs)) return e;
+    if (rcwa_status e = fence(from, fs)) return e;
     for (; lf < Lmax && off[lf + 1] <= upto; ++lf)
       if (rcwa_status e = layer_step(act[lf], S11l + (size_t)off[lf] * n2, S12l + (size_t)off[lf] * n2))
         return e;
@@ -991,28 +1059,73 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   // eigensolver) in two halves, and the fold steps of the first half's
   // layers start while the second half is formed.
   const int parts = overlap ? 2 : 1;
-  for (int l0 = 0; l0 < TL; l0 += Bl) {
-    const int b = std::min(Bl, TL - l0);
+  // The layer stage over layer matrices [l, l + m) on c.stream with work
+  // buffers wk (phase and ev_group: layer_stage's).
+  auto stage = [&](int l, int m, LayerWork& wk, int phase, int ev_group) -> rcwa_status {
     // sweep points: inverses formed per row above; [[mu]] = I is its own
     const ZCMat none_m{nullptr, 0, 0};
-    const ZCMat erci_pre = erci_all ? ZCMat{erci_all + (size_t)l0 * N2, N, N2} : none_m;
-    const ZCMat urci_pre = (plan && mu_unit) ? ZCMat{urc + (size_t)l0 * N2, N, N2}
-                           : urci_all        ? ZCMat{urci_all + (size_t)l0 * N2, N, N2}
+    const ZCMat erci_pre = erci_all ? ZCMat{erci_all + (size_t)l * N2, N, N2} : none_m;
+    const ZCMat urci_pre = (plan && mu_unit) ? ZCMat{urc + (size_t)l * N2, N, N2}
+                           : urci_all        ? ZCMat{urci_all + (size_t)l * N2, N, N2}
                                              : none_m;
-    st = layer_stage(c, N, b, ZCMat{erc + (size_t)l0 * N2, N, N2}, ZCMat{urc + (size_t)l0 * N2, N, N2},
-                     dKx + (size_t)l0 * N, dKy + (size_t)l0 * N, dv0 + (size_t)l0 * 4 * N,
-                     dk0t + l0, lam2 + (size_t)l0 * n, lam + (size_t)l0 * n,
-                     ilam + (size_t)l0 * n, Xd + (size_t)l0 * n,
-                     ZMat{S11l + (size_t)l0 * n2, n, n2}, ZMat{S12l + (size_t)l0 * n2, n, n2}, w,
-                     info_eig + l0, info_sing + l0, erci_pre, urci_pre, dcond + l0, parts,
-                     [&](int done) { return queue_fold(l0 + done); });
+    hipStream_t from = c.stream;
+    return layer_stage(c, N, m, ZCMat{erc + (size_t)l * N2, N, N2},
+                       ZCMat{urc + (size_t)l * N2, N, N2}, dKx + (size_t)l * N,
+                       dKy + (size_t)l * N, dv0 + (size_t)l * 4 * N, dk0t + l,
+                       lam2 + (size_t)l * n, lam + (size_t)l * n, ilam + (size_t)l * n,
+                       Xd + (size_t)l * n, ZMat{S11l + (size_t)l * n2, n, n2},
+                       ZMat{S12l + (size_t)l * n2, n, n2}, wk, info_eig + l, info_sing + l,
+                       erci_pre, urci_pre, dcond + l, parts,
+                       [&, l, from](int done) { return queue_fold(l + done, from); }, phase,
+                       ev_group);
+  };
+  for (int l0 = 0; l0 < TL; l0 += Bl) {
+    const int b = std::min(Bl, TL - l0);
+    if (halves) {
+      // The batch as two halves: the second half's stage runs on stream3 and
+      // starts once the first half's QR iteration is queued, so that its
+      // matrices and Hessenberg reduction are formed beside that iteration;
+      // the first half's S-matrices are then formed beside the second half's
+      // QR iteration.  The fold steps of a layer whose matrices straddle the
+      // halves follow both.
+      const int h1 = half1(b), h2 = b - h1;
+      hipStream_t s3 = c.stream3;
+      st = stage(l0, h1, w, 1, 0);
+      // c.ev[9]: recorded after the first half's Hessenberg reduction, before
+      // its QR iteration
+      if (!st) {
+        const hipError_t he = hipStreamWaitEvent(s3, c.ev[9], 0);
+        if (he != hipSuccess)
+          st = (rcwa_status)rcwa_internal_hip_fail(he, "hipStreamWaitEvent(s3, c.ev[9], 0)",
+                                                   __FILE__, __LINE__);
+      }
+      if (!st) {
+        c.stream = s3;
+        st = stage(l0 + h1, h2, w2, 1, 1);
+        c.stream = s;
+      }
+      if (!st) st = stage(l0, h1, w, 2, 0);
+      if (!st) st = fence(s, fs);
+      if (!st) {
+        c.stream = s3;
+        st = stage(l0 + h1, h2, w2, 2, 1);
+        c.stream = s;
+      }
+      if (!st) st = fence(s3, s);
+      if (st) {
+        (void)hipStreamSynchronize(s3);
+        return st;
+      }
+    } else {
+      st = stage(l0, b, w, 0, 0);
+    }
     if (st) return st;
     if (progress) {
       CKH(hipStreamSynchronize(s));
       fprintf(stderr, "[rcwa] layer matrices %d/%d done\n", l0 + b, TL);
       fflush(stderr);
     }
-    if ((st = queue_fold(l0 + b)) != RCWA_OK) return st;
+    if ((st = queue_fold(l0 + b, s)) != RCWA_OK) return st;
   }
   CKH(hipEventRecord(c.ev[4], s));
   // * S_trn: only the columns

@@ -22,6 +22,9 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the fold of the batched solve, beside the layer stage
+  // the second half of a small layer batch, staggered behind the first (the
+  // batched solve's layer stage)
+  hipStream_t stream3 = nullptr;
   size_t budget = 0;
   char* arena = nullptr;
   void* split_ws = nullptr;  // operand slices of the split-precision GEMM
@@ -38,7 +41,7 @@ struct Ctx {
   rcwa_precision precision = RCWA_FP64;
   rcwa_stats stats{};
   std::vector<std::complex<double>> last_cols;
-  hipEvent_t ev[12] = {};
+  hipEvent_t ev[16] = {};
   bool ev_on = false;
   bool time_eig = false;  // synchronise after each layer sub-batch to time the eigensolver
 
@@ -81,12 +84,17 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
 // receives sqrt(n) ||W^-1||_F per layer, a bound on cond_2(W).  After the
 // eigensolver the batch may be finished in `parts` sub-batches, on_part(done)
 // running after each with the number of layers whose S-matrices are formed.
+// The stage runs on c.stream.  phase (eig_batched's): 1 returns once the
+// eigensolver's QR iteration is queued, 2 finishes the stage (same
+// arguments), 0 does both.  The eigensolver's timing events (c.time_eig) are
+// c.ev[8 + 4 * ev_group] to c.ev[11 + 4 * ev_group], ev_group 0 or 1.
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,
                         ZCMat erci_pre = ZCMat{nullptr, 0, 0},
                         ZCMat urci_pre = ZCMat{nullptr, 0, 0}, double* cond = nullptr,
-                        int parts = 1, const std::function<rcwa_status(int)>& on_part = nullptr);
+                        int parts = 1, const std::function<rcwa_status(int)>& on_part = nullptr,
+                        int phase = 0, int ev_group = 0);
 
 rcwa_status star_product(Ctx& c, int n, int batch, zc* A[4], const ZCMat B[4], zc* tmp[5],
                          void* gjws, int* info_sing, bool keep_slots);
