@@ -15,7 +15,8 @@ GPU tests rebuild from rcwa-cuda_amd/workloads.py with the same arguments.
                   stack on 1024 x 1024 grids, lambda 400 nm, theta 30 deg
 
 The oracle is single-threaded and naive-loop: a C2 wavelength takes about
-a quarter of an hour of one core, the C5 pair of layers about two hours.
+a quarter of an hour of one core, the C5 pair of layers about four hours
+(13,829 s measured).
 Run once, in the build container:
 
     python tests/golden/make_fixtures.py c2 14 60 120 199   # one process each
