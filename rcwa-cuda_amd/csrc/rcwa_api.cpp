@@ -68,7 +68,6 @@ rcwa_status Ctx::init(int dev, size_t bud) {
   CKH(hipSetDevice(dev));
   CKH(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   CKH(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
-  CKH(hipStreamCreateWithFlags(&stream3, hipStreamNonBlocking));
   size_t fr = 0, tot = 0;
   CKH(hipMemGetInfo(&fr, &tot));
   budget = bud ? bud : (size_t)(0.8 * (double)fr);
@@ -702,7 +701,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   // halves on two streams (FP64 form; the split form's products share one
   // slice workspace).  RCWA_LAYER_HALVES=0 turns this off, =1 on for any
   // single batch of two or more matrices.
-  bool halves = c.stream2 && c.stream3 && c.precision == RCWA_FP64 && Bl >= TL && TL >= 2;
+  bool halves = c.stream2 && c.precision == RCWA_FP64 && Bl >= TL && TL >= 2;
   if (const char* he = getenv("RCWA_LAYER_HALVES"))
     halves = halves && he[0] == '1';
   else
@@ -720,6 +719,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
         halves = false;
     }
   }
+  if (halves && !c.stream3) CKH(hipStreamCreateWithFlags(&c.stream3, hipStreamNonBlocking));
   rcwa_status st = c.reserve(need);
   if (st) return st;
 

@@ -23,7 +23,10 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the fold of the batched solve, beside the layer stage
   // the second half of a small layer batch, staggered behind the first (the
-  // batched solve's layer stage)
+  // batched solve's layer stage).  Created on first use: a process has four
+  // hardware queues, and streams beyond them share one and run in order, so
+  // contexts that never take this path (the server's two workers, two
+  // streams each) must not hold it.
   hipStream_t stream3 = nullptr;
   size_t budget = 0;
   char* arena = nullptr;
