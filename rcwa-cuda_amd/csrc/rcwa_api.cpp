@@ -1082,12 +1082,13 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   for (int l0 = 0; l0 < TL; l0 += Bl) {
     const int b = std::min(Bl, TL - l0);
     if (halves) {
-      // The batch as two halves: the second half's stage runs on stream3 and
-      // starts once the first half's QR iteration is queued, so that its
-      // matrices and Hessenberg reduction are formed beside that iteration;
-      // the first half's S-matrices are then formed beside the second half's
-      // QR iteration.  The fold steps of a layer whose matrices straddle the
-      // halves follow both.
+      // The batch as two halves: the second half's stage runs on stream3
+      // behind the first half's Hessenberg reduction, so that its matrices
+      // and Hessenberg reduction are formed beside the first half's QR
+      // iteration; the first half's S-matrices are then formed beside the
+      // second half's QR iteration.  The host queues both halves up to their
+      // QR iterations before it waits on either.  The fold steps of a layer
+      // whose matrices straddle the halves follow both.
       const int h1 = half1(b), h2 = b - h1;
       hipStream_t s3 = c.stream3;
       st = stage(l0, h1, w, 1, 0);
