@@ -88,6 +88,8 @@ void Ctx::release() {
   arena_cap = 0;
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (stream4) (void)hipStreamDestroy(stream4);
+  stream4 = nullptr;
   if (stream3) (void)hipStreamDestroy(stream3);
   stream3 = nullptr;
   if (stream2) (void)hipStreamDestroy(stream2);
@@ -107,6 +109,7 @@ rcwa_status Ctx::reserve(size_t bytes) {
     CKH(hipStreamSynchronize(stream));
     if (stream2) CKH(hipStreamSynchronize(stream2));
     if (stream3) CKH(hipStreamSynchronize(stream3));
+    if (stream4) CKH(hipStreamSynchronize(stream4));
     if (arena) CKH(hipFree(arena));
     arena = nullptr;
     arena_cap = 0;
@@ -126,6 +129,7 @@ rcwa_status Ctx::set_precision(rcwa_precision p) {
     CKH(hipStreamSynchronize(stream));
     if (stream2) CKH(hipStreamSynchronize(stream2));
     if (stream3) CKH(hipStreamSynchronize(stream3));
+    if (stream4) CKH(hipStreamSynchronize(stream4));
     if (split_ws) CKH(hipFree(split_ws));
     if (split_ws2) CKH(hipFree(split_ws2));
     split_ws = split_ws2 = nullptr;
@@ -697,8 +701,8 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   size_t need = chunk_bytes(N, TL, Lmax, Nx, Ny, hx, hy, host_grids, G, Bl) + extra;
   // The QR iteration runs one workgroup per matrix: a layer batch of at most
   // CUs / 2 matrices (n >= 256) leaves most of the chip idle while it runs,
-  // so such a batch, when it is the only one, is formed as two staggered
-  // halves on two streams (FP64 form; the split form's products share one
+  // so such a batch, when it is the only one, is formed as staggered groups
+  // on two or three streams (FP64 form; the split form's products share one
   // slice workspace).  RCWA_LAYER_HALVES=0 turns this off, =1 on for any
   // single batch of two or more matrices.
   bool halves = c.stream2 && c.precision == RCWA_FP64 && Bl >= TL && TL >= 2;
@@ -708,10 +712,17 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     halves = halves && 2 * N >= 256 && TL <= c.cus / 2;
   // the first half (a first half of 30 or 40 % measured the same as 50 % on
   // the 41 x 41-harmonic, 50-layer stack)
-  auto half1 = [](int b) { return (b + 1) / 2; };
+  // Three groups measured 14.97-14.99 s against 15.21-15.35 s for two on
+  // the 41 x 41-harmonic, 50-layer stack (alternating runs on one box);
+  // RCWA_LAYER_GROUPS=2 keeps two.
+  int ngroups = !halves ? 1 : TL >= 3 ? 3 : 2;
+  if (const char* ge = getenv("RCWA_LAYER_GROUPS"))
+    if (ge[0] == '2' && ngroups > 2) ngroups = 2;
+  auto gsize = [&](int g) { return (int)((long long)TL * (g + 1) / ngroups - (long long)TL * g / ngroups); };
   if (halves) {
-    const size_t one = layer_work_bytes(N, Bl),
-                 two = layer_work_bytes(N, half1(TL)) + layer_work_bytes(N, TL - half1(TL));
+    size_t two = 0;
+    for (int g = 0; g < ngroups; ++g) two += layer_work_bytes(N, gsize(g));
+    const size_t one = layer_work_bytes(N, Bl);
     if (two > one) {
       if (need + (two - one) <= c.budget)
         need += two - one;
@@ -720,6 +731,7 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     }
   }
   if (halves && !c.stream3) CKH(hipStreamCreateWithFlags(&c.stream3, hipStreamNonBlocking));
+  if (ngroups > 2 && !c.stream4) CKH(hipStreamCreateWithFlags(&c.stream4, hipStreamNonBlocking));
   rcwa_status st = c.reserve(need);
   if (st) return st;
 
@@ -782,10 +794,9 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   int* dcolidx = (int*)c.take(4 * sizeof(int));
   int* drflag = (int*)c.take((size_t)std::max(G, 1) * sizeof(int));
   void* gjf = c.take(block_inverse_ws_bytes(n, std::max(G, 1)));  // the fold's inverses (and the row inverses)
-  LayerWork w, w2;
+  LayerWork w, wg[3];
   if (halves) {
-    w.carve(c, N, half1(TL));
-    w2.carve(c, N, TL - half1(TL));
+    for (int g = 0; g < ngroups; ++g) wg[g].carve(c, N, gsize(g));
   } else {
     w.carve(c, N, Bl);
   }
@@ -1082,39 +1093,40 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
   for (int l0 = 0; l0 < TL; l0 += Bl) {
     const int b = std::min(Bl, TL - l0);
     if (halves) {
-      // The batch as two halves: the second half's stage runs on stream3
-      // behind the first half's Hessenberg reduction, so that its matrices
-      // and Hessenberg reduction are formed beside the first half's QR
-      // iteration; the first half's S-matrices are then formed beside the
-      // second half's QR iteration.  The host queues both halves up to their
-      // QR iterations before it waits on either.  The fold steps of a layer
-      // whose matrices straddle the halves follow both.
-      const int h1 = half1(b), h2 = b - h1;
-      hipStream_t s3 = c.stream3;
-      st = stage(l0, h1, w, 1, 0);
-      // c.ev[9]: recorded after the first half's Hessenberg reduction, before
-      // its QR iteration
-      if (!st) {
-        const hipError_t he = hipStreamWaitEvent(s3, c.ev[9], 0);
-        if (he != hipSuccess)
-          st = (rcwa_status)rcwa_internal_hip_fail(he, "hipStreamWaitEvent(s3, c.ev[9], 0)",
-                                                   __FILE__, __LINE__);
-      }
-      if (!st) {
-        c.stream = s3;
-        st = stage(l0 + h1, h2, w2, 1, 1);
+      // The batch as staggered groups: group g's stage runs on its own
+      // stream (c.stream, stream3, stream4) behind group g-1's Hessenberg
+      // reduction, so that its matrices and Hessenberg reduction are formed
+      // beside the earlier groups' QR iterations; each group's S-matrices are
+      // then formed beside the later groups' QR iterations.  The host queues
+      // every group up to its QR iteration before it waits on any.  The fold
+      // stream follows each group in turn, so the fold steps of a layer whose
+      // matrices straddle two groups follow both.
+      hipStream_t gs[3] = {s, c.stream3, c.stream4};
+      int g0[4] = {0, 0, 0, 0};
+      for (int g = 0; g < ngroups; ++g) g0[g + 1] = g0[g] + gsize(g);
+      for (int g = 0; g < ngroups && !st; ++g) {
+        if (g > 0) {
+          // c.ev[9 + 4 (g - 1)]: recorded after the previous group's
+          // Hessenberg reduction, before its QR iteration
+          const hipError_t he = hipStreamWaitEvent(gs[g], c.ev[9 + 4 * (g - 1)], 0);
+          if (he != hipSuccess) {
+            st = (rcwa_status)rcwa_internal_hip_fail(he, "hipStreamWaitEvent", __FILE__, __LINE__);
+            break;
+          }
+        }
+        c.stream = gs[g];
+        st = stage(l0 + g0[g], g0[g + 1] - g0[g], wg[g], 1, g);
         c.stream = s;
       }
-      if (!st) st = stage(l0, h1, w, 2, 0);
-      if (!st) st = fence(s, fs);
-      if (!st) {
-        c.stream = s3;
-        st = stage(l0 + h1, h2, w2, 2, 1);
+      for (int g = 0; g < ngroups && !st; ++g) {
+        c.stream = gs[g];
+        st = stage(l0 + g0[g], g0[g + 1] - g0[g], wg[g], 2, g);
         c.stream = s;
+        if (!st) st = fence(gs[g], fs);
       }
-      if (!st) st = fence(s3, s);
+      for (int g = 1; g < ngroups && !st; ++g) st = fence(gs[g], s);
       if (st) {
-        (void)hipStreamSynchronize(s3);
+        for (int g = 1; g < ngroups; ++g) (void)hipStreamSynchronize(gs[g]);
         return st;
       }
     } else {

@@ -22,12 +22,13 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;  // the fold of the batched solve, beside the layer stage
-  // the second half of a small layer batch, staggered behind the first (the
-  // batched solve's layer stage).  Created on first use: a process has four
-  // hardware queues, and streams beyond them share one and run in order, so
-  // contexts that never take this path (the server's two workers, two
-  // streams each) must not hold it.
+  // the second and third groups of a small layer batch, each staggered behind
+  // the one before (the batched solve's layer stage).  Created on first use:
+  // a process has four hardware queues, and streams beyond them share one
+  // and run in order, so contexts that never take this path (the server's
+  // two workers, two streams each) must not hold them.
   hipStream_t stream3 = nullptr;
+  hipStream_t stream4 = nullptr;
   size_t budget = 0;
   char* arena = nullptr;
   void* split_ws = nullptr;  // operand slices of the split-precision GEMM
@@ -44,7 +45,7 @@ struct Ctx {
   rcwa_precision precision = RCWA_FP64;
   rcwa_stats stats{};
   std::vector<std::complex<double>> last_cols;
-  hipEvent_t ev[16] = {};
+  hipEvent_t ev[20] = {};
   bool ev_on = false;
   bool time_eig = false;  // synchronise after each layer sub-batch to time the eigensolver
 
@@ -90,7 +91,7 @@ size_t chunk_bytes(int N, int TL, int Lmax, int Nx, int Ny, int hx, int hy, bool
 // The stage runs on c.stream.  phase (eig_batched's): 1 returns once the
 // eigensolver's QR iteration is queued, 2 finishes the stage (same
 // arguments), 0 does both.  The eigensolver's timing events (c.time_eig) are
-// c.ev[8 + 4 * ev_group] to c.ev[11 + 4 * ev_group], ev_group 0 or 1.
+// c.ev[8 + 4 * ev_group] to c.ev[11 + 4 * ev_group], ev_group 0 to 2.
 rcwa_status layer_stage(Ctx& c, int N, int Bl, ZCMat erc, ZCMat urc, const zc* Kx, const zc* Ky,
                         const zc* v0, const double* k0t, zc* lam2, zc* lam, zc* ilam, zc* X,
                         ZMat S11, ZMat S12, LayerWork& w, int* info_eig, int* info_sing,

@@ -1,12 +1,13 @@
-"""A single small layer batch formed as two staggered halves on two streams
-(the second half's stage starts once the first half's QR iteration is
-queued) against the same batch formed in one piece (RCWA_LAYER_HALVES=0),
-on the same chunks: one structure of five layers at 15 x 15 harmonics
-(n = 450, where the halves are the default), three structures of three
-layers (nine layer matrices: layer 1's three straddle the halves, 5 + 4),
-and the halves forced on at 11 x 11 harmonics (n = 242).  Each matrix goes
-through the same kernels either way, so the two agree to rounding; the
-lossless case also checks R + T = 1."""
+"""A single small layer batch formed as staggered groups on their own streams
+(each group's stage starts behind the previous group's Hessenberg
+reduction; three groups by default, two with RCWA_LAYER_GROUPS=2) against
+the same batch formed in one piece (RCWA_LAYER_HALVES=0), on the same
+chunks: one structure of five layers at 15 x 15 harmonics (n = 450, where
+the groups are the default; 1 + 2 + 2), three structures of three layers
+(nine layer matrices, 3 + 3 + 3, or 5 + 4 with two groups: layer 1's three
+straddle the halves), and the groups forced on at 11 x 11 harmonics
+(n = 242).  Each matrix goes through the same kernels either way, so the
+two agree to rounding; the lossless cases also check R + T = 1."""
 import numpy as np
 import pytest
 
@@ -24,25 +25,31 @@ def _problems(seed, h, count, L, lossless):
             for _ in range(count)]
 
 
-@pytest.mark.parametrize("h,count,L,lossless,force", [
-    (7, 1, 5, False, False),
-    (7, 3, 3, True, False),
-    (5, 2, 4, False, True),
+@pytest.mark.parametrize("h,count,L,lossless,force,groups", [
+    (7, 1, 5, False, False, 3),
+    (7, 3, 3, True, False, 3),
+    (7, 3, 3, True, False, 2),
+    (5, 2, 4, False, True, 3),
 ])
-def test_layer_halves_match_one_batch(ctx, monkeypatch, h, count, L, lossless, force):
+def test_layer_halves_match_one_batch(ctx, monkeypatch, h, count, L, lossless, force, groups):
     probs = _problems(2000 + 10 * h + count + L, h, count, L, lossless)
     if force:
         monkeypatch.setenv("RCWA_LAYER_HALVES", "1")
     else:
         monkeypatch.delenv("RCWA_LAYER_HALVES", raising=False)
+    if groups == 2:
+        monkeypatch.setenv("RCWA_LAYER_GROUPS", "2")
+    else:
+        monkeypatch.delenv("RCWA_LAYER_GROUPS", raising=False)
     two = ctx.solve(probs)
+    monkeypatch.delenv("RCWA_LAYER_GROUPS", raising=False)
     monkeypatch.setenv("RCWA_LAYER_HALVES", "0")
     one = ctx.solve(probs)
     monkeypatch.delenv("RCWA_LAYER_HALVES")
     for a, b in zip(two, one):
         d = max(float(np.abs(np.asarray(x) - np.asarray(y)).max()) for x, y in zip(a, b))
-        print("two halves against one batch (h = %d, %d x %d layers): max diff %.2e"
-              % (h, count, L, d))
+        print("%d groups against one batch (h = %d, %d x %d layers): max diff %.2e"
+              % (groups, h, count, L, d))
         assert d < 1e-10, d
         if lossless:
             e = float(np.abs(a[2].sum(axis=1) - 1.0).max())
@@ -51,7 +58,8 @@ def test_layer_halves_match_one_batch(ctx, monkeypatch, h, count, L, lossless, f
 
 
 def test_layer_halves_timing_counts_both(ctx, monkeypatch):
-    """With the eigensolver timers on, both halves are counted."""
+    """With the eigensolver timers on, every group is counted (four layer
+    matrices: groups of 1, 1 and 2)."""
     probs = _problems(2100, 7, 1, 4, False)
     monkeypatch.delenv("RCWA_LAYER_HALVES", raising=False)
     ctx.set_timing(True)
@@ -62,5 +70,5 @@ def test_layer_halves_timing_counts_both(ctx, monkeypatch):
     finally:
         ctx.set_timing(False)
     print("eigensolver launches %d, matrices %d" % (st["n_eig_launches"], st["n_eig_matrices"]))
-    assert st["n_eig_launches"] == 2
+    assert st["n_eig_launches"] == 3
     assert st["n_eig_matrices"] == 4
