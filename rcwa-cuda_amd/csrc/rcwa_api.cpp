@@ -712,12 +712,15 @@ static rcwa_status run_chunk(Ctx& c, std::vector<Job*> jobs) {
     halves = halves && 2 * N >= 256 && TL <= c.cus / 2;
   // the first half (a first half of 30 or 40 % measured the same as 50 % on
   // the 41 x 41-harmonic, 50-layer stack)
-  // Three groups measured 14.97-14.99 s against 15.21-15.35 s for two on
-  // the 41 x 41-harmonic, 50-layer stack (alternating runs on one box);
-  // RCWA_LAYER_GROUPS=2 keeps two.
-  int ngroups = !halves ? 1 : TL >= 3 ? 3 : 2;
+  // Two groups by default; RCWA_LAYER_GROUPS=3 takes three (a fourth
+  // stream).  Three measured 14.97-14.99 s against 15.21-15.35 s for two on
+  // the 41 x 41-harmonic, 50-layer stack, but slowed per-call solves of
+  // small requests (up to 25 x 25 harmonics, 1-30 layers): 2.37 against
+  // 2.52 solves/s alone in a process, 1.73 against 2.72 after a server had
+  // run in it (four streams leave no hardware queue to spare).
+  int ngroups = halves ? 2 : 1;
   if (const char* ge = getenv("RCWA_LAYER_GROUPS"))
-    if (ge[0] == '2' && ngroups > 2) ngroups = 2;
+    if (ge[0] == '3' && halves && TL >= 3) ngroups = 3;
   auto gsize = [&](int g) { return (int)((long long)TL * (g + 1) / ngroups - (long long)TL * g / ngroups); };
   if (halves) {
     size_t two = 0;

@@ -1,11 +1,11 @@
 """A single small layer batch formed as staggered groups on their own streams
 (each group's stage starts behind the previous group's Hessenberg
-reduction; three groups by default, two with RCWA_LAYER_GROUPS=2) against
+reduction; two groups by default, three with RCWA_LAYER_GROUPS=3) against
 the same batch formed in one piece (RCWA_LAYER_HALVES=0), on the same
 chunks: one structure of five layers at 15 x 15 harmonics (n = 450, where
-the groups are the default; 1 + 2 + 2), three structures of three layers
-(nine layer matrices, 3 + 3 + 3, or 5 + 4 with two groups: layer 1's three
-straddle the halves), and the groups forced on at 11 x 11 harmonics
+the groups are the default; 3 + 2, or 1 + 2 + 2), three structures of three
+layers (nine layer matrices, 5 + 4: layer 1's three straddle the halves, or
+3 + 3 + 3), and the groups forced on at 11 x 11 harmonics
 (n = 242).  Each matrix goes through the same kernels either way, so the
 two agree to rounding; the lossless cases also check R + T = 1."""
 import numpy as np
@@ -26,10 +26,11 @@ def _problems(seed, h, count, L, lossless):
 
 
 @pytest.mark.parametrize("h,count,L,lossless,force,groups", [
+    (7, 1, 5, False, False, 2),
     (7, 1, 5, False, False, 3),
-    (7, 3, 3, True, False, 3),
     (7, 3, 3, True, False, 2),
-    (5, 2, 4, False, True, 3),
+    (7, 3, 3, True, False, 3),
+    (5, 2, 4, False, True, 2),
 ])
 def test_layer_halves_match_one_batch(ctx, monkeypatch, h, count, L, lossless, force, groups):
     probs = _problems(2000 + 10 * h + count + L, h, count, L, lossless)
@@ -37,8 +38,8 @@ def test_layer_halves_match_one_batch(ctx, monkeypatch, h, count, L, lossless, f
         monkeypatch.setenv("RCWA_LAYER_HALVES", "1")
     else:
         monkeypatch.delenv("RCWA_LAYER_HALVES", raising=False)
-    if groups == 2:
-        monkeypatch.setenv("RCWA_LAYER_GROUPS", "2")
+    if groups == 3:
+        monkeypatch.setenv("RCWA_LAYER_GROUPS", "3")
     else:
         monkeypatch.delenv("RCWA_LAYER_GROUPS", raising=False)
     two = ctx.solve(probs)
@@ -59,7 +60,7 @@ def test_layer_halves_match_one_batch(ctx, monkeypatch, h, count, L, lossless, f
 
 def test_layer_halves_timing_counts_both(ctx, monkeypatch):
     """With the eigensolver timers on, every group is counted (four layer
-    matrices: groups of 1, 1 and 2)."""
+    matrices: groups of 2 and 2)."""
     probs = _problems(2100, 7, 1, 4, False)
     monkeypatch.delenv("RCWA_LAYER_HALVES", raising=False)
     ctx.set_timing(True)
@@ -70,5 +71,5 @@ def test_layer_halves_timing_counts_both(ctx, monkeypatch):
     finally:
         ctx.set_timing(False)
     print("eigensolver launches %d, matrices %d" % (st["n_eig_launches"], st["n_eig_matrices"]))
-    assert st["n_eig_launches"] == 3
+    assert st["n_eig_launches"] == 2
     assert st["n_eig_matrices"] == 4
